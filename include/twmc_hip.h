@@ -1,0 +1,173 @@
+/*
+ * twmc_hip.h -- C ABI of libtwmc_hip.so, the MI355X (gfx950) Truncated Wigner
+ * Monte Carlo trajectory engine.
+ *
+ * This is the drop-in boundary for the reference's trajectory solvers.  A
+ * context owns one parameter set on one GPU (the reference's
+ * TWMCSystemData + one Solver instance); a batch is a set of trajectories
+ * that share t_start, t_end and the initial-condition mode (the reference's
+ * vector of TWMCTaskData passed to one Solver::Compute call).
+ *
+ *   reference interface                                   replaced by
+ *   Solver::Setup  (src/Base/Solver.hpp:22,
+ *     TWMCBaseSolver.cpp:24-92, TWMCLiebSolver.cpp:29-104) twmc_create
+ *   Solver::Compute (src/Base/Solver.hpp:23,
+ *     TWMCBaseSolver.cpp:94-283, TWMCLiebSolver.cpp:109-299) twmc_run, or
+ *                                                          twmc_batch_begin /
+ *                                                          _advance / _sync / _read
+ *   Solver::ApproximateComputationProgress
+ *     (src/Base/Solver.hpp:25, TWMCBaseSolver.cpp:293-295) twmc_progress
+ *   SolverGPU::SetupGPU (src/Base/SolverGPU.hpp:20)        `device` argument of twmc_create
+ *   Solver::~Solver (WorkerThread.cpp:30)                 twmc_destroy
+ *   std::cerr diagnostics (TWMCBaseSolver.cpp:268-274)    int return codes + twmc_last_error
+ *   NoisyMatrix::Generate (src/Base/NoisyMatrix.cpp:118-160)
+ *                                                         twmc_noisy_matrix (host)
+ *
+ * Conventions (the reference's, unchanged):
+ *   - lattice row-major nx x ny; a chain is nx = 1, ny = N (TWMCSystemData.cpp:25-37)
+ *   - complex arrays are interleaved (re, im) doubles
+ *   - hopping FFT:  h = IDFT(K . DFT(beta)) / N,
+ *                   K(i,j) = -i 2 J ([nx!=1] cos(2 pi i/nx) + [ny!=1] cos(2 pi j/ny))
+ *   - hopping matrix: h = -i (beta . C), C real N x N (TWMCLiebSolver.cpp:262)
+ *   - drift a = dt [ (rsl + i U (|beta|^2 - 1)) beta + h + i F ],
+ *     rsl = -i omega - gamma/2 (Base) or -i (detuning - E) - gamma/2 (Lieb)
+ *   - beta <- beta + a + sqrt(gamma dt / 4) xi, xi complex N(0,1) from Philox
+ * No function throws; all return TWMC_OK (0) or a negative TWMC_ERR_* code.
+ * A context is not thread-safe; use one per host thread.
+ */
+#ifndef TWMC_HIP_H
+#define TWMC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TWMC_ABI_VERSION 1
+
+enum {
+  TWMC_OK = 0,
+  TWMC_ERR_INVALID = -1,     /* bad argument or unsupported combination */
+  TWMC_ERR_HIP = -2,         /* HIP runtime error (no GPU, launch failure, ...) */
+  TWMC_ERR_NOMEM = -3,       /* device or host allocation failed */
+  TWMC_ERR_STATE = -4,       /* call out of order (e.g. advance without begin) */
+  TWMC_ERR_UNSUPPORTED = -5  /* lattice or size no kernel handles */
+};
+
+enum { TWMC_NOISE_NONE = 0, TWMC_NOISE_UNIFORM = 1, TWMC_NOISE_GAUSSIAN = 2 };
+enum { TWMC_HOP_FFT = 0, TWMC_HOP_MATRIX = 1 };
+enum { TWMC_ONSITE_OMEGA = 0, TWMC_ONSITE_DETUNING_MINUS_E = 1 };
+enum { TWMC_INIT_SETTINGS = 0, TWMC_INIT_FIXEDPOINT = 1, TWMC_INIT_PREVIOUS = 2 };
+enum {
+  TWMC_KERNEL_AUTO = 0,     /* fastest kernel that supports the parameters */
+  TWMC_KERNEL_GENERIC = 1,  /* LDS-resident mixed-radix kernel, any supported size */
+  TWMC_KERNEL_FUSED64 = 2,  /* register-resident 64x64 FP32 kernel */
+  TWMC_KERNEL_MFMA = 3      /* matrix hopping as BF16x3 MFMA GEMM (FP32 state) */
+};
+/* output mask bits for twmc_batch_desc.outputs */
+enum {
+  TWMC_OUT_OBS = 1u,        /* per-frame ensemble sums (always cheap) */
+  TWMC_OUT_TRAJ = 2u,       /* full per-trajectory frames (the reference's "traj") */
+  TWMC_OUT_FINAL = 4u       /* final state of every trajectory */
+};
+
+/* A site-resolved complex parameter with optional per-trajectory disorder
+ * (the reference's NoisyMatrix: base value plus Gaussian N(0,1) or uniform
+ * U(0,1) noise scaled by amp, real and imaginary parts separately). */
+typedef struct twmc_matrix {
+  const double* base;     /* 2N interleaved, or NULL: value[] on every site */
+  double value[2];
+  int32_t noise_type;     /* TWMC_NOISE_* */
+  const double* amp;      /* 2N interleaved, or NULL: amp_value[] on every site */
+  double amp_value[2];
+} twmc_matrix;
+
+typedef struct twmc_params {
+  int32_t nx, ny;         /* lattice after 1D normalisation */
+  int32_t hopping;        /* TWMC_HOP_* */
+  int32_t onsite;         /* TWMC_ONSITE_* */
+  int32_t precision;      /* 32 or 64 */
+  int32_t kernel;         /* TWMC_KERNEL_* */
+  double dt, gamma, J, detuning;
+  twmc_matrix omega;      /* omega (Base) or E (Lieb) */
+  twmc_matrix U, F, beta_init, beta_sigma;
+  const double* coupling; /* TWMC_HOP_MATRIX: N x N real, row-major */
+  uint64_t seed;          /* Philox key */
+  uint64_t frame_steps;   /* steps between stored frames */
+} twmc_params;
+
+typedef struct twmc_batch_desc {
+  const uint64_t* traj_ids;   /* n_traj trajectory ids (Philox counter) */
+  uint64_t n_traj;
+  int32_t init_mode;          /* TWMC_INIT_* */
+  int32_t store_initial;      /* store the state before the first step */
+  const double* beta_prev;    /* TWMC_INIT_PREVIOUS: n_traj x N complex (host) */
+  uint64_t step0;             /* absolute index of the first step (resume) */
+  double t_start, t_end;
+  uint32_t outputs;           /* TWMC_OUT_* mask */
+} twmc_batch_desc;
+
+typedef struct twmc_batch_info {
+  uint64_t n_frames;          /* frames this batch will store */
+  uint64_t n_steps;           /* Euler steps it will take */
+  uint64_t n_segments;        /* launches of the step kernel */
+  uint64_t segments_done;
+  double t_final;             /* t after the loop, as the reference reports it */
+  int32_t kernel;             /* TWMC_KERNEL_* chosen */
+  int32_t truncated;          /* 1 if the time loop ends before n_frames */
+} twmc_batch_info;
+
+/* Host buffers, caller-allocated; NULL entries are skipped.
+ * Site and k arrays are row-major nx x ny; frames are in storing order. */
+typedef struct twmc_output {
+  double* traj;        /* n_traj x n_frames x N complex */
+  double* beta_final;  /* n_traj x N complex */
+  double* sum_n;       /* n_frames x N : sum over trajectories of |beta_i|^2 */
+  double* sum_n2;      /* n_frames x N : sum of |beta_i|^4 */
+  double* sum_nk;      /* n_frames x N : sum of |DFT(beta)_k|^2 (unnormalised DFT) */
+  double* sum_beta;    /* n_frames x N complex : sum of beta_i */
+  double* traj_m2;     /* n_traj x n_frames : site mean of |beta_i|^2 */
+  double* traj_m4;     /* n_traj x n_frames : site mean of |beta_i|^4 */
+} twmc_output;
+
+typedef struct twmc_stats {
+  uint64_t step_launches;    /* step-kernel launches timed */
+  double step_kernel_ms;     /* summed HIP-event time of those launches */
+  uint64_t site_steps;       /* trajectories x sites x steps they advanced */
+  double reduce_kernel_ms;
+  uint64_t reduce_launches;
+  int32_t kernel;            /* kernel used by the last batch */
+  int32_t grid;              /* workgroups of the step kernel */
+  int32_t block;             /* threads per workgroup */
+  int32_t lds_bytes;         /* dynamic LDS per workgroup */
+} twmc_stats;
+
+int twmc_abi_version(void);
+const char* twmc_last_error(const void* ctx);  /* ctx may be NULL: last global error */
+
+int twmc_create(const twmc_params* p, int device, void** ctx_out);
+void twmc_destroy(void* ctx);
+
+int twmc_batch_begin(void* ctx, const twmc_batch_desc* d);
+int twmc_batch_info_get(void* ctx, twmc_batch_info* info);
+int twmc_batch_advance(void* ctx, uint64_t n_segments);   /* asynchronous */
+int twmc_batch_sync(void* ctx);
+int twmc_batch_read(void* ctx, twmc_output* out);         /* synchronises */
+int twmc_run(void* ctx, const twmc_batch_desc* d, twmc_output* out);
+
+int twmc_set_timing(void* ctx, int enable);   /* HIP events around step launches */
+int twmc_query(void* ctx, twmc_stats* st);
+int twmc_reset_stats(void* ctx);
+float twmc_progress(const void* ctx);         /* fraction of the batch's steps enqueued and done */
+
+/* Host-side NoisyMatrix realisation with the kernels' Philox layout (tag as
+ * in twmc_rng.h).  Used for one shared disorder realisation. */
+int twmc_noisy_matrix(const twmc_matrix* m, uint64_t n, uint64_t seed, uint64_t traj,
+                      uint32_t tag, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,746 @@
+// twmc_capi.hip -- C ABI of libtwmc_hip.so (declared in include/twmc_hip.h).
+//
+// A context holds one parameter set on one GPU (the reference's
+// TWMCSystemData plus a Solver after Setup, src/TWMC/Solvers/TWMCBaseSolver.cpp:24-92);
+// a batch is the vector of tasks of one Solver::Compute call
+// (TWMCBaseSolver.cpp:94-283), run as a sequence of segments: one launch of
+// the step kernel per stored frame, each followed by the reduction of the
+// frame's ensemble sums.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/twmc_hip.h"
+#include "twmc_internal.h"
+#include "twmc_rng.h"
+
+using namespace twmc;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Segment {
+  int capture;     // accumulate the state before the first step
+  int frame;       // frame index of that capture
+  uint64_t steps;  // steps in the segment
+};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+struct Ctx {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  twmc_params p{};
+  int n = 0;
+  int precision = 64;
+  size_t es = 16;  // bytes per complex element
+  int kernel = TWMC_KERNEL_GENERIC;
+  int noisy_params = 0;
+  double dt4 = 0.0;
+  RadixPlan px{}, py{};
+  // host copies of the site-resolved inputs (2N doubles each)
+  std::vector<double> base[5], amp[5];
+  int noise[5] = {0, 0, 0, 0, 0};
+  // device-resident constants
+  DevBuf d_base[5], d_amp[5], d_twx, d_twy, d_kcx, d_kcy, d_cscp, d_cscr, d_cscv, d_perm_site,
+      d_perm_k;
+  // per-batch buffers
+  DevBuf d_ids, d_beta, d_rsl, d_U, d_F, d_partials, d_obs, d_m2, d_m4, d_traj, d_prev;
+  int csc_nnz = 0;
+  // batch state
+  bool batch_active = false;
+  twmc_batch_desc desc{};
+  uint64_t n_traj = 0;
+  uint64_t n_frames = 0, n_steps = 0;
+  double t_final = 0.0;
+  int truncated = 0;
+  std::vector<Segment> segs;
+  size_t next_seg = 0;
+  uint64_t steps_done = 0;
+  int grid = 0, block = 256;
+  size_t lds = 0;
+  // timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_step, ev_red;
+  twmc_stats stats{};
+};
+
+int fail(Ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(Ctx* c, hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return fail(c, TWMC_ERR_HIP, m);
+}
+
+#define HIPCHK(c, expr)                                 \
+  do {                                                  \
+    hipError_t e_ = (expr);                             \
+    if (e_ != hipSuccess) return hip_fail(c, e_, #expr); \
+  } while (0)
+
+bool make_plan(int len, RadixPlan& P) {
+  P.len = len;
+  P.npass = 0;
+  int m = len;
+  const int order[] = {8, 4, 2, 3, 5};
+  for (int r : order) {
+    while (m % r == 0) {
+      if (P.npass >= kMaxRadixPasses) return false;
+      P.radix[P.npass++] = r;
+      m /= r;
+    }
+  }
+  return m == 1;
+}
+
+void expand_matrix(const twmc_matrix& m, int n, std::vector<double>& base, std::vector<double>& amp,
+                   int& noise) {
+  base.assign(2 * (size_t)n, 0.0);
+  amp.assign(2 * (size_t)n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    base[2 * i] = m.base ? m.base[2 * i] : m.value[0];
+    base[2 * i + 1] = m.base ? m.base[2 * i + 1] : m.value[1];
+  }
+  noise = m.noise_type;
+  if (noise != TWMC_NOISE_NONE) {
+    for (int i = 0; i < n; ++i) {
+      amp[2 * i] = m.amp ? m.amp[2 * i] : m.amp_value[0];
+      amp[2 * i + 1] = m.amp ? m.amp[2 * i + 1] : m.amp_value[1];
+    }
+  }
+}
+
+template <typename T>
+std::vector<unsigned char> pack_complex(const std::vector<double>& v) {
+  std::vector<unsigned char> out(v.size() * sizeof(T));
+  T* o = reinterpret_cast<T*>(out.data());
+  for (size_t i = 0; i < v.size(); ++i) o[i] = T(v[i]);
+  return out;
+}
+
+hipError_t upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
+  hipError_t e = b.ensure(bytes);
+  if (e != hipSuccess || bytes == 0) return e;
+  return hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s);
+}
+
+// The reference's time loop (TWMCBaseSolver.cpp:221-274) run on the host to
+// fix the segment plan: frames at i_step % frame_steps == 0, loop while
+// t <= t_end with t accumulated in FP64, break after the last frame.
+void plan_segments(Ctx* c) {
+  const twmc_batch_desc& d = c->desc;
+  uint64_t fs = c->p.frame_steps;
+  double dt_obs = c->p.dt * (double)fs;
+  uint64_t nf = (uint64_t)(size_t)nearbyint((d.t_end - d.t_start) / dt_obs);
+  if (d.store_initial) nf++;
+  std::vector<uint64_t> cap;  // step index of each capture
+  uint64_t i_frame = 0, i_step = 0;
+  if (d.store_initial && i_frame < nf) {
+    cap.push_back(0);
+    i_frame++;
+  }
+  double t = d.t_start;
+  while (t <= d.t_end) {
+    t += c->p.dt;
+    i_step++;
+    if ((i_step % fs == 0) && i_frame < nf) {
+      cap.push_back(i_step);
+      i_frame++;
+      if (i_frame == nf) break;
+    }
+  }
+  c->truncated = i_frame < nf ? 1 : 0;
+  c->n_frames = i_frame;
+  c->n_steps = i_step;
+  c->t_final = t;
+  // one segment per stored frame: it starts at the capture step and runs to
+  // the next one; steps before the first capture form a segment without one
+  c->segs.clear();
+  std::vector<std::pair<uint64_t, int>> starts;
+  if (cap.empty() || cap[0] > 0) starts.push_back({0, -1});
+  for (size_t k = 0; k < cap.size(); ++k) starts.push_back({cap[k], (int)k});
+  for (size_t k = 0; k < starts.size(); ++k) {
+    uint64_t end = (k + 1 < starts.size()) ? starts[k + 1].first : i_step;
+    Segment sg{starts[k].second >= 0 ? 1 : 0, starts[k].second, end - starts[k].first};
+    if (sg.capture || sg.steps > 0) c->segs.push_back(sg);
+  }
+}
+
+int choose_kernel(Ctx* c) {
+  const twmc_params& p = c->p;
+  bool fused_ok = fused64_supported(p.nx, p.ny, p.hopping, c->precision, c->noisy_params ? 1 : 0);
+  size_t lds = generic_lds_bytes(p.nx, p.ny, c->precision);
+  bool generic_ok = generic_pairs_per_thread(c->n, 256) > 0 && lds <= 160 * 1024 &&
+                    (p.hopping == TWMC_HOP_MATRIX || (c->px.npass >= 0 && c->py.npass >= 0));
+  int k = p.kernel;
+  if (k == TWMC_KERNEL_AUTO) k = fused_ok ? TWMC_KERNEL_FUSED64 : TWMC_KERNEL_GENERIC;
+  if (k == TWMC_KERNEL_FUSED64 && !fused_ok)
+    return fail(c, TWMC_ERR_UNSUPPORTED,
+                "fused 64x64 kernel needs nx = ny = 64, FFT hopping, FP32 and no per-trajectory disorder");
+  if (k == TWMC_KERNEL_GENERIC && !generic_ok)
+    return fail(c, TWMC_ERR_UNSUPPORTED, "lattice too large for the LDS-resident kernel");
+  if (k == TWMC_KERNEL_MFMA)
+    return fail(c, TWMC_ERR_UNSUPPORTED, "MFMA matrix-hopping kernel is not available in this build");
+  c->kernel = k;
+  return TWMC_OK;
+}
+
+int release_batch(Ctx* c) {
+  c->batch_active = false;
+  c->segs.clear();
+  c->next_seg = 0;
+  return TWMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int twmc_abi_version(void) { return TWMC_ABI_VERSION; }
+
+const char* twmc_last_error(const void* ctx) {
+  if (ctx) return static_cast<const Ctx*>(ctx)->err.c_str();
+  return g_last_error.c_str();
+}
+
+int twmc_create(const twmc_params* p, int device, void** ctx_out) {
+  if (!p || !ctx_out) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
+  *ctx_out = nullptr;
+  if (p->nx < 1 || p->ny < 1) return fail(nullptr, TWMC_ERR_INVALID, "nx and ny must be >= 1");
+  if (p->precision != 32 && p->precision != 64)
+    return fail(nullptr, TWMC_ERR_INVALID, "precision must be 32 or 64");
+  if (p->frame_steps < 1) return fail(nullptr, TWMC_ERR_INVALID, "frame_steps must be >= 1");
+  if (!(p->dt > 0.0)) return fail(nullptr, TWMC_ERR_INVALID, "dt must be > 0");
+  if (p->hopping != TWMC_HOP_FFT && p->hopping != TWMC_HOP_MATRIX)
+    return fail(nullptr, TWMC_ERR_INVALID, "unknown hopping mode");
+  if (p->hopping == TWMC_HOP_MATRIX && !p->coupling)
+    return fail(nullptr, TWMC_ERR_INVALID, "matrix hopping needs a coupling matrix");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return fail(nullptr, TWMC_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(nullptr, TWMC_ERR_INVALID, "device index out of range");
+
+  Ctx* c = new Ctx();
+  c->device = device;
+  c->p = *p;
+  c->n = p->nx * p->ny;
+  c->precision = p->precision;
+  c->es = p->precision == 64 ? 16 : 8;
+  c->dt4 = std::sqrt(p->gamma * p->dt / 4.0);  // TWMCBaseSolver.cpp:210
+  const twmc_matrix* mats[5] = {&p->omega, &p->U, &p->F, &p->beta_init, &p->beta_sigma};
+  for (int k = 0; k < 5; ++k) {
+    if (mats[k]->noise_type < 0 || mats[k]->noise_type > 2) {
+      delete c;
+      return fail(nullptr, TWMC_ERR_INVALID, "unknown noise type");
+    }
+    expand_matrix(*mats[k], c->n, c->base[k], c->amp[k], c->noise[k]);
+  }
+  c->p.omega.base = c->p.U.base = c->p.F.base = c->p.beta_init.base = c->p.beta_sigma.base = nullptr;
+  c->p.omega.amp = c->p.U.amp = c->p.F.amp = c->p.beta_init.amp = c->p.beta_sigma.amp = nullptr;
+  c->noisy_params = (c->noise[0] || c->noise[1] || c->noise[2]) ? 1 : 0;
+
+  if (p->hopping == TWMC_HOP_FFT) {
+    if (!make_plan(p->nx, c->px) || !make_plan(p->ny, c->py)) {
+      delete c;
+      return fail(nullptr, TWMC_ERR_UNSUPPORTED, "FFT lengths must factor into 2, 3, 5");
+    }
+  } else {
+    c->px.len = c->py.len = 0;
+    c->px.npass = c->py.npass = 0;
+  }
+  int rc = choose_kernel(c);
+  if (rc != TWMC_OK) {
+    std::string m = c->err;
+    delete c;
+    return fail(nullptr, rc, m);
+  }
+
+  e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(nullptr, e, "hipSetDevice");
+  }
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(nullptr, e, "hipStreamCreate");
+  }
+
+  // constant tables, computed as the reference computes them
+  // (K: TWMCBaseSolver.cpp:78-88; twiddles: exp(-2 pi i m / L) in long double)
+  auto tw_table = [](int L) {
+    std::vector<double> t(2 * (size_t)std::max(L, 1));
+    for (int m = 0; m < L; ++m) {
+      long double a = -2.0L * 3.14159265358979323846264338327950288L * m / L;
+      t[2 * m] = (double)cosl(a);
+      t[2 * m + 1] = (double)sinl(a);
+    }
+    return t;
+  };
+  double fx = (p->nx == 1) ? 0.0 : 1.0, fy = (p->ny == 1) ? 0.0 : 1.0;
+  std::vector<double> kcx(p->nx), kcy(p->ny);
+  for (int i = 0; i < p->nx; ++i) kcx[i] = fx * cos(2.0 * M_PI / double(p->nx) * double(i));
+  for (int j = 0; j < p->ny; ++j) kcy[j] = fy * cos(2.0 * M_PI / double(p->ny) * double(j));
+  std::vector<double> twx = tw_table(p->nx), twy = tw_table(p->ny);
+
+  hipStream_t s = c->stream;
+  int fail_rc = TWMC_OK;
+  auto up = [&](DevBuf& b, const std::vector<double>& v, bool is_complex_or_real_T) {
+    (void)is_complex_or_real_T;
+    std::vector<unsigned char> pk = c->precision == 64 ? pack_complex<double>(v) : pack_complex<float>(v);
+    hipError_t ee = upload(b, pk.data(), pk.size(), s);
+    if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "upload constant table");
+    return ee;
+  };
+  up(c->d_twx, twx, true);
+  up(c->d_twy, twy, true);
+  up(c->d_kcx, kcx, false);
+  up(c->d_kcy, kcy, false);
+  for (int k = 0; k < 5 && fail_rc == TWMC_OK; ++k) {
+    hipError_t ee = upload(c->d_base[k], c->base[k].data(), c->base[k].size() * 8, s);
+    if (ee == hipSuccess) ee = upload(c->d_amp[k], c->amp[k].data(), c->amp[k].size() * 8, s);
+    if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "upload parameters");
+  }
+  if (fail_rc == TWMC_OK && p->hopping == TWMC_HOP_MATRIX) {
+    // column-compressed nonzeros of C: h_j = sum_i beta_i C_ij
+    std::vector<int> ptr(c->n + 1, 0), row;
+    std::vector<double> val;
+    for (int j = 0; j < c->n; ++j) {
+      for (int i = 0; i < c->n; ++i) {
+        double v = p->coupling[(size_t)i * c->n + j];
+        if (v != 0.0) {
+          row.push_back(i);
+          val.push_back(v);
+        }
+      }
+      ptr[j + 1] = (int)row.size();
+    }
+    c->csc_nnz = (int)row.size();
+    if (row.empty()) {
+      row.push_back(0);
+      val.push_back(0.0);
+    }
+    hipError_t ee = upload(c->d_cscp, ptr.data(), ptr.size() * sizeof(int), s);
+    if (ee == hipSuccess) ee = upload(c->d_cscr, row.data(), row.size() * sizeof(int), s);
+    if (ee == hipSuccess) up(c->d_cscv, val, false);
+    if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "upload coupling");
+  }
+  if (fail_rc == TWMC_OK && c->kernel == TWMC_KERNEL_FUSED64) {
+    std::vector<int> ps(4096), pk(4096);
+    fused64_perms(ps.data(), pk.data());
+    hipError_t ee = upload(c->d_perm_site, ps.data(), ps.size() * sizeof(int), s);
+    if (ee == hipSuccess) ee = upload(c->d_perm_k, pk.data(), pk.size() * sizeof(int), s);
+    if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "upload permutations");
+  }
+  if (fail_rc == TWMC_OK) {
+    hipError_t ee = hipStreamSynchronize(s);
+    if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "hipStreamSynchronize");
+  }
+  if (fail_rc != TWMC_OK) {
+    std::string m = c->err;
+    twmc_destroy(c);
+    return fail(nullptr, fail_rc, m);
+  }
+  *ctx_out = c;
+  return TWMC_OK;
+}
+
+void twmc_destroy(void* ctx) {
+  if (!ctx) return;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& pr : c->ev_step) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  for (auto& pr : c->ev_red) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  DevBuf* bufs[] = {&c->d_twx, &c->d_twy, &c->d_kcx, &c->d_kcy, &c->d_cscp, &c->d_cscr,
+                    &c->d_cscv, &c->d_perm_site, &c->d_perm_k, &c->d_ids, &c->d_beta, &c->d_rsl,
+                    &c->d_U, &c->d_F, &c->d_partials, &c->d_obs, &c->d_m2, &c->d_m4, &c->d_traj,
+                    &c->d_prev};
+  for (DevBuf* b : bufs) b->release();
+  for (int k = 0; k < 5; ++k) {
+    c->d_base[k].release();
+    c->d_amp[k].release();
+  }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
+  if (!ctx || !d) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (d->n_traj == 0 || !d->traj_ids) return fail(c, TWMC_ERR_INVALID, "empty trajectory batch");
+  if (d->n_traj > (uint64_t)INT32_MAX) return fail(c, TWMC_ERR_INVALID, "batch too large");
+  if (d->init_mode < 0 || d->init_mode > 2) return fail(c, TWMC_ERR_INVALID, "unknown init mode");
+  if (d->init_mode == TWMC_INIT_PREVIOUS && !d->beta_prev)
+    return fail(c, TWMC_ERR_INVALID, "TWMC_INIT_PREVIOUS needs beta_prev");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->desc = *d;
+  c->desc.traj_ids = nullptr;
+  c->desc.beta_prev = nullptr;
+  c->n_traj = d->n_traj;
+  plan_segments(c);
+  c->next_seg = 0;
+  c->steps_done = 0;
+
+  const int n = c->n;
+  const size_t B = d->n_traj;
+  hipStream_t s = c->stream;
+  // launch geometry
+  int cus = 0;
+  HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  int per_cu = 1;
+  if (c->kernel == TWMC_KERNEL_FUSED64) {
+    c->block = 64;
+    c->lds = 0;
+    HIPCHK(c, fused64_occupancy(&per_cu));
+  } else {
+    c->block = 256;
+    c->lds = generic_lds_bytes(c->p.nx, c->p.ny, c->precision);
+    HIPCHK(c, generic_occupancy(c->precision, n, c->block, c->lds, &per_cu));
+  }
+  if (per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");
+  c->grid = (int)std::min<size_t>(B, (size_t)per_cu * cus);
+
+  size_t sets = c->noisy_params ? B : 1;
+  HIPCHK(c, upload(c->d_ids, d->traj_ids, B * sizeof(uint64_t), s));
+  HIPCHK(c, c->d_beta.ensure(B * n * c->es));
+  HIPCHK(c, c->d_rsl.ensure(sets * n * c->es));
+  HIPCHK(c, c->d_U.ensure(sets * n * c->es));
+  HIPCHK(c, c->d_F.ensure(sets * n * c->es));
+  size_t part_bytes = (size_t)c->grid * kQ * n * sizeof(double);
+  HIPCHK(c, c->d_partials.ensure(part_bytes));
+  HIPCHK(c, hipMemsetAsync(c->d_partials.p, 0, part_bytes, s));
+  size_t nf = std::max<uint64_t>(c->n_frames, 1);
+  HIPCHK(c, c->d_obs.ensure(nf * kQ * n * sizeof(double)));
+  HIPCHK(c, hipMemsetAsync(c->d_obs.p, 0, nf * kQ * n * sizeof(double), s));
+  HIPCHK(c, c->d_m2.ensure(B * nf * sizeof(double)));
+  HIPCHK(c, c->d_m4.ensure(B * nf * sizeof(double)));
+  if (d->outputs & TWMC_OUT_TRAJ) HIPCHK(c, c->d_traj.ensure(B * nf * n * c->es));
+  if (d->init_mode == TWMC_INIT_PREVIOUS)
+    HIPCHK(c, upload(c->d_prev, d->beta_prev, B * n * 2 * sizeof(double), s));
+
+  InitArgs ia{};
+  ia.n = n;
+  ia.n_traj = (int)B;
+  ia.mode = d->init_mode;
+  ia.per_traj_params = c->noisy_params;
+  ia.onsite = c->p.onsite;
+  ia.gamma = c->p.gamma;
+  ia.detuning = c->p.detuning;
+  ia.key0 = uint32_t(c->p.seed & 0xffffffffu);
+  ia.key1 = uint32_t(c->p.seed >> 32);
+  ia.traj_ids = static_cast<const uint64_t*>(c->d_ids.p);
+  for (int k = 0; k < 5; ++k) {
+    ia.base[k] = static_cast<const double*>(c->d_base[k].p);
+    ia.amp[k] = static_cast<const double*>(c->d_amp[k].p);
+    ia.noise[k] = c->noise[k];
+  }
+  ia.beta_prev = static_cast<const double*>(c->d_prev.p);
+  ia.beta = c->d_beta.p;
+  ia.rsl = c->d_rsl.p;
+  ia.U = c->d_U.p;
+  ia.F = c->d_F.p;
+  HIPCHK(c, launch_init(ia, c->precision, s));
+  c->batch_active = true;
+  return TWMC_OK;
+}
+
+int twmc_batch_info_get(void* ctx, twmc_batch_info* info) {
+  if (!ctx || !info) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c->batch_active) return fail(c, TWMC_ERR_STATE, "no active batch");
+  info->n_frames = c->n_frames;
+  info->n_steps = c->n_steps;
+  info->n_segments = c->segs.size();
+  info->segments_done = c->next_seg;
+  info->t_final = c->t_final;
+  info->kernel = c->kernel;
+  info->truncated = c->truncated;
+  return TWMC_OK;
+}
+
+int twmc_batch_advance(void* ctx, uint64_t n_segments) {
+  if (!ctx) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c->batch_active) return fail(c, TWMC_ERR_STATE, "no active batch");
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  for (uint64_t k = 0; k < n_segments && c->next_seg < c->segs.size(); ++k) {
+    const Segment& sg = c->segs[c->next_seg];
+    // a segment longer than 2^31 steps is split; captures stay on the first piece
+    uint64_t left = sg.steps;
+    bool first = true;
+    do {
+      int steps = (int)std::min<uint64_t>(left, 1u << 30);
+      StepArgs a{};
+      a.nx = c->p.nx;
+      a.ny = c->p.ny;
+      a.n = n;
+      a.hopping = c->p.hopping;
+      a.n_traj = (int)c->n_traj;
+      a.param_stride = c->noisy_params ? n : 0;
+      a.dt = c->p.dt;
+      a.dt4 = c->dt4;
+      a.inv_n = 1.0 / double(n);
+      a.kscale = -2.0 * c->p.J;
+      a.key0 = uint32_t(c->p.seed & 0xffffffffu);
+      a.key1 = uint32_t(c->p.seed >> 32);
+      a.step0 = c->desc.step0 + c->steps_done;
+      a.n_steps = steps;
+      a.capture = (first && sg.capture) ? 1 : 0;
+      a.frame = sg.frame < 0 ? 0 : sg.frame;
+      a.n_frames = (int)std::max<uint64_t>(c->n_frames, 1);
+      a.dump = (c->desc.outputs & TWMC_OUT_TRAJ) ? 1 : 0;
+      a.px = c->px;
+      a.py = c->py;
+      a.traj_ids = static_cast<const uint64_t*>(c->d_ids.p);
+      a.beta = c->d_beta.p;
+      a.rsl = c->d_rsl.p;
+      a.U = c->d_U.p;
+      a.F = c->d_F.p;
+      a.twx = c->d_twx.p;
+      a.twy = c->d_twy.p;
+      a.kcx = c->d_kcx.p;
+      a.kcy = c->d_kcy.p;
+      a.csc_ptr = static_cast<const int*>(c->d_cscp.p);
+      a.csc_row = static_cast<const int*>(c->d_cscr.p);
+      a.csc_val = c->d_cscv.p;
+      a.partials = static_cast<double*>(c->d_partials.p);
+      a.traj_m2 = static_cast<double*>(c->d_m2.p);
+      a.traj_m4 = static_cast<double*>(c->d_m4.p);
+      a.traj_out = c->d_traj.p;
+      if (a.n_steps > 0 || a.capture) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (c->timing) {
+          HIPCHK(c, hipEventCreate(&e0));
+          HIPCHK(c, hipEventCreate(&e1));
+          HIPCHK(c, hipEventRecord(e0, s));
+        }
+        if (c->kernel == TWMC_KERNEL_FUSED64) {
+          HIPCHK(c, launch_fused64(a, c->grid, s));
+        } else {
+          HIPCHK(c, launch_generic(a, c->precision, c->grid, c->block, c->lds, s));
+        }
+        if (c->timing) {
+          HIPCHK(c, hipEventRecord(e1, s));
+          c->ev_step.push_back({e0, e1});
+        }
+        c->stats.site_steps += (uint64_t)a.n_steps * (uint64_t)n * c->n_traj;
+        if (a.capture) {
+          ReduceArgs r{};
+          r.n = n;
+          r.slots = c->grid;
+          r.partials_in = static_cast<const double*>(c->d_partials.p);
+          r.partials = static_cast<double*>(c->d_partials.p);
+          r.obs = static_cast<double*>(c->d_obs.p) + (size_t)a.frame * kQ * n;
+          r.perm_site = c->kernel == TWMC_KERNEL_FUSED64 ? static_cast<const int*>(c->d_perm_site.p) : nullptr;
+          r.perm_k = c->kernel == TWMC_KERNEL_FUSED64 ? static_cast<const int*>(c->d_perm_k.p) : nullptr;
+          hipEvent_t r0 = nullptr, r1 = nullptr;
+          if (c->timing) {
+            HIPCHK(c, hipEventCreate(&r0));
+            HIPCHK(c, hipEventCreate(&r1));
+            HIPCHK(c, hipEventRecord(r0, s));
+          }
+          HIPCHK(c, launch_reduce(r, s));
+          if (c->timing) {
+            HIPCHK(c, hipEventRecord(r1, s));
+            c->ev_red.push_back({r0, r1});
+          }
+        }
+      }
+      c->steps_done += (uint64_t)steps;
+      left -= (uint64_t)steps;
+      first = false;
+    } while (left > 0);
+    c->next_seg++;
+  }
+  return TWMC_OK;
+}
+
+int twmc_batch_sync(void* ctx) {
+  if (!ctx) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
+  Ctx* c = static_cast<Ctx*>(ctx);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TWMC_OK;
+}
+
+int twmc_batch_read(void* ctx, twmc_output* out) {
+  if (!ctx || !out) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c->batch_active) return fail(c, TWMC_ERR_STATE, "no active batch");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t n = c->n, B = c->n_traj, nf = c->n_frames;
+  if (nf > 0 && (out->sum_n || out->sum_n2 || out->sum_nk || out->sum_beta)) {
+    std::vector<double> obs(nf * kQ * n);
+    HIPCHK(c, hipMemcpy(obs.data(), c->d_obs.p, obs.size() * 8, hipMemcpyDeviceToHost));
+    for (size_t f = 0; f < nf; ++f) {
+      const double* o = obs.data() + f * kQ * n;
+      if (out->sum_n) std::memcpy(out->sum_n + f * n, o, n * 8);
+      if (out->sum_n2) std::memcpy(out->sum_n2 + f * n, o + n, n * 8);
+      if (out->sum_nk) std::memcpy(out->sum_nk + f * n, o + 2 * n, n * 8);
+      if (out->sum_beta)
+        for (size_t i = 0; i < n; ++i) {
+          out->sum_beta[2 * (f * n + i)] = o[3 * n + i];
+          out->sum_beta[2 * (f * n + i) + 1] = o[4 * n + i];
+        }
+    }
+  }
+  if (nf > 0 && out->traj_m2) HIPCHK(c, hipMemcpy(out->traj_m2, c->d_m2.p, B * nf * 8, hipMemcpyDeviceToHost));
+  if (nf > 0 && out->traj_m4) HIPCHK(c, hipMemcpy(out->traj_m4, c->d_m4.p, B * nf * 8, hipMemcpyDeviceToHost));
+  auto to_f64 = [&](const DevBuf& b, size_t count, double* dst) -> int {
+    if (c->precision == 64) {
+      HIPCHK(c, hipMemcpy(dst, b.p, count * 16, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> tmp(count * 2);
+      HIPCHK(c, hipMemcpy(tmp.data(), b.p, count * 8, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < count * 2; ++i) dst[i] = tmp[i];
+    }
+    return TWMC_OK;
+  };
+  if (out->traj) {
+    if (!(c->desc.outputs & TWMC_OUT_TRAJ))
+      return fail(c, TWMC_ERR_STATE, "trajectory frames were not requested (TWMC_OUT_TRAJ)");
+    if (nf > 0) {
+      int rc = to_f64(c->d_traj, B * nf * n, out->traj);
+      if (rc) return rc;
+    }
+  }
+  if (out->beta_final) {
+    int rc = to_f64(c->d_beta, B * n, out->beta_final);
+    if (rc) return rc;
+  }
+  return TWMC_OK;
+}
+
+int twmc_run(void* ctx, const twmc_batch_desc* d, twmc_output* out) {
+  int rc = twmc_batch_begin(ctx, d);
+  if (rc) return rc;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  rc = twmc_batch_advance(ctx, c->segs.size());
+  if (rc) return rc;
+  if (out) rc = twmc_batch_read(ctx, out);
+  else rc = twmc_batch_sync(ctx);
+  return rc;
+}
+
+int twmc_set_timing(void* ctx, int enable) {
+  if (!ctx) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
+  static_cast<Ctx*>(ctx)->timing = enable != 0;
+  return TWMC_OK;
+}
+
+int twmc_query(void* ctx, twmc_stats* st) {
+  if (!ctx || !st) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
+  Ctx* c = static_cast<Ctx*>(ctx);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto& pr : c->ev_step) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, pr.first, pr.second));
+    c->stats.step_kernel_ms += ms;
+    c->stats.step_launches++;
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  c->ev_step.clear();
+  for (auto& pr : c->ev_red) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, pr.first, pr.second));
+    c->stats.reduce_kernel_ms += ms;
+    c->stats.reduce_launches++;
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  c->ev_red.clear();
+  c->stats.kernel = c->kernel;
+  c->stats.grid = c->grid;
+  c->stats.block = c->block;
+  c->stats.lds_bytes = (int)c->lds;
+  *st = c->stats;
+  return TWMC_OK;
+}
+
+int twmc_reset_stats(void* ctx) {
+  if (!ctx) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
+  Ctx* c = static_cast<Ctx*>(ctx);
+  twmc_stats z{};
+  twmc_query(ctx, &z);
+  c->stats = twmc_stats{};
+  return TWMC_OK;
+}
+
+float twmc_progress(const void* ctx) {
+  if (!ctx) return 0.f;
+  const Ctx* c = static_cast<const Ctx*>(ctx);
+  if (!c->batch_active || c->n_steps == 0) return 0.f;
+  return float(double(c->steps_done) / double(c->n_steps));
+}
+
+int twmc_noisy_matrix(const twmc_matrix* m, uint64_t n, uint64_t seed, uint64_t traj, uint32_t tag,
+                      double* out) {
+  if (!m || !out) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
+  uint32_t k0 = uint32_t(seed & 0xffffffffu), k1 = uint32_t(seed >> 32);
+  for (uint64_t i = 0; i < n; ++i) {
+    double re = m->base ? m->base[2 * i] : m->value[0];
+    double im = m->base ? m->base[2 * i + 1] : m->value[1];
+    if (m->noise_type != TWMC_NOISE_NONE) {
+      u32x4 x = philox4x32_10(make_counter(traj, tag, 0u, (uint32_t)(i >> 1)), k0, k1);
+      uint32_t w0 = (i & 1) ? x.z : x.x, w1 = (i & 1) ? x.w : x.y;
+      double zr, zi;
+      if (m->noise_type == TWMC_NOISE_GAUSSIAN) {
+        box_muller_f64(w0, w1, zr, zi);
+      } else {
+        zr = (double(w0) + 0.5) * 2.3283064365386963e-10;
+        zi = (double(w1) + 0.5) * 2.3283064365386963e-10;
+      }
+      double ar = m->amp ? m->amp[2 * i] : m->amp_value[0];
+      double ai = m->amp ? m->amp[2 * i + 1] : m->amp_value[1];
+      re += zr * ar;
+      im += zi * ai;
+    }
+    out[2 * i] = re;
+    out[2 * i + 1] = im;
+  }
+  return TWMC_OK;
+}
+
+}  // extern "C"

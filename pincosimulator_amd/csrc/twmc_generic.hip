@@ -1,0 +1,480 @@
+// twmc_generic.hip -- LDS-resident TWMC step kernel for any lattice whose
+// FFT lengths factor into 2, 3, 4, 5, 8 (and for matrix hopping), FP32 or
+// FP64.  One workgroup advances one trajectory at a time; a persistent grid
+// walks the batch.  This kernel is the FP64 parity path and the fallback for
+// shapes the register-resident kernel does not cover.
+//
+// Per step (src/TWMC/Solvers/TWMCBaseSolver.cpp:232-256):
+//   beta -> LDS, forward DFT (rows, then columns; Stockham, ping-pong),
+//   K product, backward DFT, /N, drift + Philox increment, update in registers.
+// Matrix hopping (src/TWMC/Solvers/TWMCLiebSolver.cpp:259-267):
+//   h_j = -i sum_i beta_i C_ij over the stored nonzeros of column j.
+#include "twmc_cplx.h"
+#include "twmc_internal.h"
+#include "twmc_rng.h"
+
+namespace twmc {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <int S, typename T>
+__device__ __forceinline__ cpx<T> twiddle(const cpx<T>* tw, int m) {
+  cpx<T> w = tw[m];
+  return S < 0 ? w : mk(w.re, -w.im);
+}
+
+// One Stockham pass of radix R over `batch` transforms of length L laid out
+// with element stride es and batch stride bs.  Butterfly j reads
+// in[j + r L/R], applies exp(S 2 pi i r (j mod Ns)/(Ns R)), and writes
+// out[(j/Ns) Ns R + (j mod Ns) + q Ns].
+template <int S, int R, typename T>
+__device__ __forceinline__ void stockham_pass(const cpx<T>* __restrict__ src, cpx<T>* __restrict__ dst,
+                                              int L, int Ns, int batch, int bs, int es,
+                                              const cpx<T>* tw) {
+  const int LR = L / R;
+  const int nbf = batch * LR;
+  const int tstep = L / (Ns * R);
+  for (int w = threadIdx.x; w < nbf; w += blockDim.x) {
+    int b = w / LR;
+    int j = w - b * LR;
+    int k = j % Ns;
+    const cpx<T>* in = src + b * bs;
+    cpx<T>* out = dst + b * bs;
+    cpx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      v[r] = in[(j + r * LR) * es];
+      if (r > 0 && k > 0) v[r] = cmul(v[r], twiddle<S>(tw, r * k * tstep));
+    }
+    if constexpr (R == 2) dft2<S>(v);
+    if constexpr (R == 3) dft3<S>(v);
+    if constexpr (R == 4) dft4<S>(v);
+    if constexpr (R == 5) dft5<S>(v);
+    if constexpr (R == 8) dft8<S>(v);
+    int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) out[(base + q * Ns) * es] = v[q];
+  }
+}
+
+// Full 1D transforms along one dimension; returns the buffer holding the result.
+template <int S, typename T>
+__device__ cpx<T>* fft_dim(cpx<T>* src, cpx<T>* dst, const RadixPlan& P, int batch, int bs, int es,
+                           const cpx<T>* tw) {
+  int Ns = 1;
+  for (int ip = 0; ip < P.npass; ++ip) {
+    int R = P.radix[ip];
+    switch (R) {
+      case 8: stockham_pass<S, 8>(src, dst, P.len, Ns, batch, bs, es, tw); break;
+      case 4: stockham_pass<S, 4>(src, dst, P.len, Ns, batch, bs, es, tw); break;
+      case 2: stockham_pass<S, 2>(src, dst, P.len, Ns, batch, bs, es, tw); break;
+      case 3: stockham_pass<S, 3>(src, dst, P.len, Ns, batch, bs, es, tw); break;
+      default: stockham_pass<S, 5>(src, dst, P.len, Ns, batch, bs, es, tw); break;
+    }
+    __syncthreads();
+    cpx<T>* t = src;
+    src = dst;
+    dst = t;
+    Ns *= R;
+  }
+  return src;
+}
+
+template <int S, typename T>
+__device__ cpx<T>* fft2d(cpx<T>* a, cpx<T>* b, const StepArgs& g, const cpx<T>* twx, const cpx<T>* twy) {
+  // rows: length ny, contiguous; then columns: length nx, stride ny
+  cpx<T>* cur = fft_dim<S>(a, b, g.py, g.nx, g.ny, 1, twy);
+  if (g.nx > 1) {
+    cpx<T>* other = (cur == a) ? b : a;
+    cur = fft_dim<S>(cur, other, g.px, g.ny, 1, g.ny, twx);
+  }
+  return cur;
+}
+
+template <typename T>
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, T& re, T& im) {
+  if constexpr (sizeof(T) == 8) {
+    box_muller_f64(a, b, re, im);
+  } else {
+    box_muller_f32(a, b, re, im);
+  }
+}
+
+__device__ __forceinline__ double warp_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename T, int PPT>
+__global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = g.n;
+  double* red = reinterpret_cast<double*>(smem);  // 8 doubles: per-wave partial sums
+  cpx<T>* bufA = reinterpret_cast<cpx<T>*>(smem + 64);
+  cpx<T>* bufB = bufA + n;
+  cpx<T>* twx = bufB + n;
+  cpx<T>* twy = twx + g.nx;
+  T* kcx = reinterpret_cast<T*>(twy + g.ny);
+  T* kcy = kcx + g.nx;
+
+  for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+    twx[i] = reinterpret_cast<const cpx<T>*>(g.twx)[i];
+    kcx[i] = reinterpret_cast<const T*>(g.kcx)[i];
+  }
+  for (int i = threadIdx.x; i < g.ny; i += blockDim.x) {
+    twy[i] = reinterpret_cast<const cpx<T>*>(g.twy)[i];
+    kcy[i] = reinterpret_cast<const T*>(g.kcy)[i];
+  }
+  __syncthreads();
+
+  const int npairs = (n + 1) / 2;
+  double* part = g.partials + (size_t)blockIdx.x * kQ * n;
+  const T dt = T(g.dt), dt4 = T(g.dt4), kscale = T(g.kscale);
+
+  for (int b = blockIdx.x; b < g.n_traj; b += gridDim.x) {
+    const uint64_t traj = g.traj_ids[b];
+    cpx<T>* gbeta = reinterpret_cast<cpx<T>*>(g.beta) + (size_t)b * n;
+    const size_t poff = (size_t)(g.param_stride ? b : 0) * n;
+    const cpx<T>* rsl = reinterpret_cast<const cpx<T>*>(g.rsl) + poff;
+    const cpx<T>* Up = reinterpret_cast<const cpx<T>*>(g.U) + poff;
+    const cpx<T>* Fp = reinterpret_cast<const cpx<T>*>(g.F) + poff;
+
+    cpx<T> beta[2 * PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      int p = threadIdx.x + kBlock * j;
+      if (p < npairs) {
+        beta[2 * j] = gbeta[2 * p];
+        if (2 * p + 1 < n) beta[2 * j + 1] = gbeta[2 * p + 1];
+      }
+    }
+
+    const int iters = g.n_steps > 0 ? g.n_steps : (g.capture ? 1 : 0);
+    for (int s = 0; s < iters; ++s) {
+      // stage beta in LDS
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) {
+        int p = threadIdx.x + kBlock * j;
+        if (p < npairs) {
+          bufA[2 * p] = beta[2 * j];
+          if (2 * p + 1 < n) bufA[2 * p + 1] = beta[2 * j + 1];
+        }
+      }
+      __syncthreads();
+      cpx<T>* cur = bufA;
+      if (g.hopping == 0) cur = fft2d<-1>(bufA, bufB, g, twx, twy);
+
+      if (s == 0 && g.capture) {
+        // ensemble sums of the state before this step (a stored frame)
+        if (g.hopping == 0) {
+          for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            cpx<T> f = cur[i];
+            part[2 * n + i] += double(f.re) * double(f.re) + double(f.im) * double(f.im);
+          }
+        }
+        double m2 = 0.0, m4 = 0.0;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+          int p = threadIdx.x + kBlock * j;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            int i = 2 * p + h;
+            if (p < npairs && i < n) {
+              cpx<T> z = beta[2 * j + h];
+              double nn = double(z.re) * double(z.re) + double(z.im) * double(z.im);
+              part[i] += nn;
+              part[n + i] += nn * nn;
+              part[3 * n + i] += double(z.re);
+              part[4 * n + i] += double(z.im);
+              m2 += nn;
+              m4 += nn * nn;
+              if (g.dump) {
+                cpx<T>* out = reinterpret_cast<cpx<T>*>(g.traj_out) +
+                              ((size_t)b * g.n_frames + g.frame) * n;
+                out[i] = z;
+              }
+            }
+          }
+        }
+        m2 = warp_sum(m2);
+        m4 = warp_sum(m4);
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if (lane == 0) {
+          red[2 * wave] = m2;
+          red[2 * wave + 1] = m4;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          double a2 = 0.0, a4 = 0.0;
+          for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            a2 += red[2 * w];
+            a4 += red[2 * w + 1];
+          }
+          g.traj_m2[(size_t)b * g.n_frames + g.frame] = a2 / double(n);
+          g.traj_m4[(size_t)b * g.n_frames + g.frame] = a4 / double(n);
+        }
+        __syncthreads();
+      }
+      if (s >= g.n_steps) break;
+
+      if (g.hopping == 0) {
+        // K product: K = i kscale (fx cos kx + fy cos ky) (TWMCBaseSolver.cpp:80-88, :240)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+          int kx = i / g.ny, ky = i - kx * g.ny;
+          T kim = kscale * (kcx[kx] + kcy[ky]);
+          cpx<T> f = cur[i];
+          cur[i] = mk(-kim * f.im, kim * f.re);
+        }
+        __syncthreads();
+        cpx<T>* other = (cur == bufA) ? bufB : bufA;
+        cur = fft2d<+1>(cur, other, g, twx, twy);
+      } else {
+        // h = -i (beta . C) over the nonzeros of each column (TWMCLiebSolver.cpp:262)
+        const T* vals = reinterpret_cast<const T*>(g.csc_val);
+        for (int jcol = threadIdx.x; jcol < n; jcol += blockDim.x) {
+          cpx<T> acc = mk(T(0), T(0));
+          for (int e = g.csc_ptr[jcol]; e < g.csc_ptr[jcol + 1]; ++e) {
+            cpx<T> z = bufA[g.csc_row[e]];
+            T c = vals[e];
+            acc = mk(acc.re + z.re * c, acc.im + z.im * c);
+          }
+          bufB[jcol] = mk(acc.im, -acc.re);  // -i acc
+        }
+        __syncthreads();
+        cur = bufB;
+      }
+
+      // drift + increment + update (TWMCBaseSolver.cpp:245-253)
+      const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) {
+        int p = threadIdx.x + kBlock * j;
+        if (p < npairs) {
+          u32x4 x = philox4x32_10(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
+          T xr[2], xi[2];
+          box_muller<T>(x.x, x.y, xr[0], xi[0]);
+          box_muller<T>(x.z, x.w, xr[1], xi[1]);
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            int i = 2 * p + h;
+            if (i < n) {
+              cpx<T> z = beta[2 * j + h];
+              cpx<T> hh = cur[i];
+              if (g.hopping == 0) {
+                if constexpr (sizeof(T) == 8) {
+                  hh = mk(hh.re / T(n), hh.im / T(n));
+                } else {
+                  hh = mk(hh.re * T(g.inv_n), hh.im * T(g.inv_n));
+                }
+              }
+              cpx<T> r = rsl[i], u = Up[i], f = Fp[i];
+              T q = z.re * z.re + z.im * z.im - T(1);
+              cpx<T> c = mk(r.re + (-u.im) * q, r.im + u.re * q);
+              cpx<T> cb = cmul(c, z);
+              cpx<T> sdr = mk(cb.re + hh.re + (-f.im), cb.im + hh.im + f.re);
+              cpx<T> a = mk(sdr.re * dt, sdr.im * dt);
+              z = mk(z.re + a.re + dt4 * xr[h], z.im + a.im + dt4 * xi[h]);
+              beta[2 * j + h] = z;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      int p = threadIdx.x + kBlock * j;
+      if (p < npairs) {
+        gbeta[2 * p] = beta[2 * j];
+        if (2 * p + 1 < n) gbeta[2 * p + 1] = beta[2 * j + 1];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- init
+// NoisyMatrix realisation of parameter `which` at site i for trajectory traj
+// (src/Base/NoisyMatrix.cpp:118-160), FP64.
+__device__ void gen_param(const InitArgs& a, int which, uint32_t tag, uint64_t traj, int i,
+                          double& re, double& im) {
+  re = a.base[which][2 * i];
+  im = a.base[which][2 * i + 1];
+  if (a.noise[which] == 0) return;
+  u32x4 x = philox4x32_10(make_counter(traj, tag, 0u, (uint32_t)(i >> 1)), a.key0, a.key1);
+  uint32_t w0 = (i & 1) ? x.z : x.x, w1 = (i & 1) ? x.w : x.y;
+  double zr, zi;
+  if (a.noise[which] == 2) {
+    box_muller_f64(w0, w1, zr, zi);
+  } else {
+    const double two_m32 = 2.3283064365386963e-10;
+    zr = (double(w0) + 0.5) * two_m32;
+    zi = (double(w1) + 0.5) * two_m32;
+  }
+  re += zr * a.amp[which][2 * i];
+  im += zi * a.amp[which][2 * i + 1];
+}
+
+template <typename T>
+__global__ void init_kernel(InitArgs a) {
+  const size_t total = (size_t)a.n_traj * a.n;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    int b = (int)(idx / a.n);
+    int i = (int)(idx - (size_t)b * a.n);
+    uint64_t traj = a.traj_ids[b];
+    cpx<T>* beta = reinterpret_cast<cpx<T>*>(a.beta) + (size_t)b * a.n;
+    if (a.mode == 0) {
+      double br, bi, sr, si;
+      gen_param(a, 3, TAG_BINIT, traj, i, br, bi);
+      gen_param(a, 4, TAG_BSIGMA, traj, i, sr, si);
+      u32x4 x = philox4x32_10(make_counter(traj, TAG_INIT, 0u, (uint32_t)(i >> 1)), a.key0, a.key1);
+      double zr, zi;
+      if (i & 1) box_muller_f64(x.z, x.w, zr, zi); else box_muller_f64(x.x, x.y, zr, zi);
+      // beta_init + sigma (.) xi, complex product (TWMCBaseSolver.cpp:126-134)
+      double pr = sr * zr - si * zi, pi = sr * zi + si * zr;
+      beta[i] = mk(T(br + pr), T(bi + pi));
+    } else if (a.mode == 1) {
+      beta[i] = mk(T(a.base[3][2 * i]), T(a.base[3][2 * i + 1]));
+    } else {
+      const double* bp = a.beta_prev + 2 * ((size_t)b * a.n + i);
+      beta[i] = mk(T(bp[0]), T(bp[1]));
+    }
+    if (a.per_traj_params || b == 0) {
+      size_t off = (size_t)(a.per_traj_params ? b : 0) * a.n + i;
+      double orr, oi, ur, ui, fr, fi;
+      gen_param(a, 0, TAG_OMEGA, traj, i, orr, oi);
+      gen_param(a, 1, TAG_U, traj, i, ur, ui);
+      gen_param(a, 2, TAG_F, traj, i, fr, fi);
+      double rr, ri;
+      if (a.onsite == 0) {
+        // rsl = -i omega - gamma/2 (TWMCBaseSolver.cpp:74)
+        rr = oi - a.gamma / 2.0;
+        ri = -orr;
+      } else {
+        // delta = -E + detuning; rsl = -i delta - gamma/2 (TWMCLiebSolver.cpp:44-45)
+        double dr = -orr + a.detuning, di = -oi;
+        rr = di - a.gamma / 2.0;
+        ri = -dr;
+      }
+      reinterpret_cast<cpx<T>*>(a.rsl)[off] = mk(T(rr), T(ri));
+      reinterpret_cast<cpx<T>*>(a.U)[off] = mk(T(ur), T(ui));
+      reinterpret_cast<cpx<T>*>(a.F)[off] = mk(T(fr), T(fi));
+    }
+  }
+}
+
+__global__ void reduce_kernel(ReduceArgs a) {
+  const int total = kQ * a.n;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int sl = 0; sl < a.slots; ++sl) {
+      size_t o = (size_t)sl * total + idx;
+      s += a.partials_in[o];
+      a.partials[o] = 0.0;
+    }
+    int q = idx / a.n, j = idx - q * a.n;
+    const int* perm = (q == 2) ? a.perm_k : a.perm_site;
+    int dst = perm ? perm[j] : j;
+    a.obs[q * a.n + dst] = s;
+  }
+}
+
+template <typename T, int PPT>
+hipError_t launch_generic_t(const StepArgs& a, int grid, int block, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL((generic_step_kernel<T, PPT>), dim3(grid), dim3(block), lds, s, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_generic_p(const StepArgs& a, int grid, int block, size_t lds, hipStream_t s) {
+  int ppt = generic_pairs_per_thread(a.n, block);
+  if (lds > 65536) {
+    hipError_t e;
+    switch (ppt) {
+      case 1: e = hipFuncSetAttribute((const void*)generic_step_kernel<T, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); break;
+      case 2: e = hipFuncSetAttribute((const void*)generic_step_kernel<T, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); break;
+      case 8: e = hipFuncSetAttribute((const void*)generic_step_kernel<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); break;
+      default: e = hipFuncSetAttribute((const void*)generic_step_kernel<T, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); break;
+    }
+    if (e != hipSuccess) return e;
+  }
+  switch (ppt) {
+    case 1: return launch_generic_t<T, 1>(a, grid, block, lds, s);
+    case 2: return launch_generic_t<T, 2>(a, grid, block, lds, s);
+    case 8: return launch_generic_t<T, 8>(a, grid, block, lds, s);
+    default: return launch_generic_t<T, 32>(a, grid, block, lds, s);
+  }
+}
+
+}  // namespace
+
+int generic_pairs_per_thread(int n, int block) {
+  int np = (n + 1) / 2;
+  int need = (np + block - 1) / block;
+  if (need <= 1) return 1;
+  if (need <= 2) return 2;
+  if (need <= 8) return 8;
+  if (need <= 32) return 32;
+  return -1;
+}
+
+size_t generic_lds_bytes(int nx, int ny, int precision) {
+  size_t es = precision == 64 ? 16 : 8;
+  size_t rs = precision == 64 ? 8 : 4;
+  size_t n = (size_t)nx * ny;
+  return 64 + 2 * n * es + (size_t)(nx + ny) * es + (size_t)(nx + ny) * rs;
+}
+
+hipError_t generic_occupancy(int precision, int n, int block, size_t lds, int* blocks_per_cu) {
+  int ppt = generic_pairs_per_thread(n, block);
+  const void* f;
+  if (precision == 64) {
+    f = ppt == 1 ? (const void*)generic_step_kernel<double, 1>
+        : ppt == 2 ? (const void*)generic_step_kernel<double, 2>
+        : ppt == 8 ? (const void*)generic_step_kernel<double, 8>
+                   : (const void*)generic_step_kernel<double, 32>;
+  } else {
+    f = ppt == 1 ? (const void*)generic_step_kernel<float, 1>
+        : ppt == 2 ? (const void*)generic_step_kernel<float, 2>
+        : ppt == 8 ? (const void*)generic_step_kernel<float, 8>
+                   : (const void*)generic_step_kernel<float, 32>;
+  }
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, block, lds);
+}
+
+hipError_t launch_generic(const StepArgs& a, int precision, int grid, int block, size_t lds,
+                          hipStream_t s) {
+  if (precision == 64) return launch_generic_p<double>(a, grid, block, lds, s);
+  return launch_generic_p<float>(a, grid, block, lds, s);
+}
+
+hipError_t launch_init(const InitArgs& a, int precision, hipStream_t s) {
+  size_t total = (size_t)a.n_traj * a.n;
+  int block = 256;
+  size_t grid = (total + block - 1) / block;
+  if (grid > 65536) grid = 65536;
+  if (grid < 1) grid = 1;
+  if (precision == 64)
+    hipLaunchKernelGGL(init_kernel<double>, dim3((unsigned)grid), dim3(block), 0, s, a);
+  else
+    hipLaunchKernelGGL(init_kernel<float>, dim3((unsigned)grid), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {
+  int total = kQ * a.n;
+  int block = 256;
+  int grid = (total + block - 1) / block;
+  hipLaunchKernelGGL(reduce_kernel, dim3(grid), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace twmc

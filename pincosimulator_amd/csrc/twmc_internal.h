@@ -1,0 +1,100 @@
+// twmc_internal.h -- launch interfaces shared between the C ABI layer
+// (twmc_capi.hip) and the kernel translation units.
+#ifndef TWMC_INTERNAL_H
+#define TWMC_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace twmc {
+
+// Number of accumulated per-site ensemble quantities per frame:
+// 0: |beta|^2   1: |beta|^4   2: |DFT(beta)_k|^2   3: Re beta   4: Im beta
+constexpr int kQ = 5;
+constexpr int kMaxRadixPasses = 16;
+
+// Factorisation of one FFT length into radices 8, 4, 2, 3, 5 (in that order).
+struct RadixPlan {
+  int len;
+  int npass;
+  int radix[kMaxRadixPasses];
+};
+
+// Everything a step kernel needs; plain data passed by value.
+struct StepArgs {
+  int nx, ny, n;             // lattice and site count
+  int hopping;               // 0: FFT, 1: sparse matrix (CSC)
+  int n_traj;                // trajectories in the batch
+  int param_stride;          // 0: shared site parameters, n: per-trajectory
+  double dt, dt4, inv_n;     // dt, sqrt(gamma dt / 4), 1 / N
+  double kscale;             // -2 J  (K = i kscale (fx cos kx + fy cos ky))
+  uint32_t key0, key1;       // Philox key (seed)
+  uint64_t step0;            // absolute index of the segment's first step
+  int n_steps;               // steps in this segment
+  int capture;               // accumulate the state before the first step
+  int frame;                 // frame index of that capture
+  int n_frames;              // frames in the batch (trajectory dump stride)
+  int dump;                  // write the captured state to traj_out
+  RadixPlan px, py;          // plans for the x (length nx) and y (length ny) FFTs
+  const uint64_t* traj_ids;  // [n_traj]
+  void* beta;                // [n_traj][n] complex (T)
+  const void* rsl;           // [sets][n] complex (T): -i omega - gamma/2 etc.
+  const void* U;             // [sets][n] complex (T)
+  const void* F;             // [sets][n] complex (T)
+  const void* twx;           // [nx] complex (T): exp(-2 pi i m / nx)
+  const void* twy;           // [ny] complex (T)
+  const void* kcx;           // [nx] (T): [nx != 1] cos(2 pi i / nx)
+  const void* kcy;           // [ny] (T)
+  const int* csc_ptr;        // hopping matrix, column pointers [n+1]
+  const int* csc_row;        // row index of each nonzero
+  const void* csc_val;       // value (T)
+  double* partials;          // [slots][kQ][n] slot-private FP64 ensemble sums
+  double* traj_m2;           // [n_traj][n_frames]
+  double* traj_m4;           // [n_traj][n_frames]
+  void* traj_out;            // [n_traj][n_frames][n] complex (T)
+};
+
+struct InitArgs {
+  int n, n_traj, mode;       // mode: 0 settings, 1 fixed point, 2 previous
+  int per_traj_params;       // write per-trajectory rsl/U/F
+  int onsite;                // 0: rsl = -i omega - gamma/2; 1: -i (detuning - E) - gamma/2
+  double gamma, detuning;
+  uint32_t key0, key1;
+  const uint64_t* traj_ids;
+  // site-resolved parameter bases and noise amplitudes (FP64, interleaved)
+  const double* base[5];     // omega/E, U, F, beta_init, beta_sigma
+  const double* amp[5];
+  int noise[5];
+  const double* beta_prev;   // mode 2
+  void* beta;                // out [n_traj][n] (T)
+  void* rsl;                 // out [sets][n] (T)
+  void* U;
+  void* F;
+};
+
+struct ReduceArgs {
+  int n, slots;
+  const double* partials_in;  // [slots][kQ][n]
+  double* partials;           // same buffer, cleared after reading
+  double* obs;                // [kQ][n] for one frame
+  const int* perm_site;       // native index -> natural index, or null
+  const int* perm_k;
+};
+
+hipError_t launch_init(const InitArgs& a, int precision, hipStream_t s);
+hipError_t launch_generic(const StepArgs& a, int precision, int grid, int block, size_t lds,
+                          hipStream_t s);
+size_t generic_lds_bytes(int nx, int ny, int precision);
+int generic_pairs_per_thread(int n, int block);
+hipError_t generic_occupancy(int precision, int n, int block, size_t lds, int* blocks_per_cu);
+hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
+
+// register-resident 64x64 FP32 kernel
+bool fused64_supported(int nx, int ny, int hopping, int precision, int param_stride);
+hipError_t launch_fused64(const StepArgs& a, int grid, hipStream_t s);
+hipError_t fused64_occupancy(int* blocks_per_cu);
+void fused64_perms(int* perm_site, int* perm_k);  // [4096] each
+
+}  // namespace twmc
+
+#endif

@@ -1,0 +1,100 @@
+// twmc_rng.h -- counter-based Wiener increments for the TWMC kernels.
+//
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed by the 64-bit run
+// seed, with the counter
+//     { site/2, step, traj & 0xffffffff, (tag << 24) | ((traj >> 32) & 0xffffff) }
+// so every increment is a pure function of (seed, trajectory id, step, site):
+// results do not depend on batch composition, launch shape or resume point.
+// Site 2p takes output words (x0, x1), site 2p+1 takes (x2, x3); each pair is
+// turned into one complex N(0,1) sample (Re, Im independent) by Box-Muller:
+//     u1 = (a + 1/2) 2^-32,  theta = b 2^-32 (revolutions),
+//     Re = sqrt(-2 ln u1) cos(2 pi theta),  Im = sqrt(-2 ln u1) sin(2 pi theta).
+// The reference draws these from std::mt19937 + normal_distribution
+// (src/TWMC/Solvers/TWMCBaseSolver.cpp:119-120, :285-291); its Re/Im draw order
+// is unspecified there, so this layout fixes one.
+#ifndef TWMC_RNG_H
+#define TWMC_RNG_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace twmc {
+
+enum : uint32_t {
+  TAG_WIENER = 0,
+  TAG_INIT = 1,
+  TAG_U = 2,
+  TAG_OMEGA = 3,
+  TAG_F = 4,
+  TAG_BINIT = 5,
+  TAG_BSIGMA = 6,
+  TAG_BOND = 7
+};
+
+struct u32x4 {
+  uint32_t x, y, z, w;
+};
+
+__host__ __device__ __forceinline__ void mulhilo32(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
+  uint64_t p = uint64_t(a) * uint64_t(b);
+  hi = uint32_t(p >> 32);
+  lo = uint32_t(p);
+}
+
+// Ten Philox rounds; the key is bumped by the Weyl constants between rounds.
+__host__ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint32_t hi0, lo0, hi1, lo1;
+    mulhilo32(0xD2511F53u, c.x, hi0, lo0);
+    mulhilo32(0xCD9E8D57u, c.z, hi1, lo1);
+    u32x4 n;
+    n.x = hi1 ^ c.y ^ k0;
+    n.y = lo1;
+    n.z = hi0 ^ c.w ^ k1;
+    n.w = lo0;
+    c = n;
+  }
+  return c;
+}
+
+__host__ __device__ __forceinline__ u32x4 make_counter(uint64_t traj, uint32_t tag, uint32_t step, uint32_t pair) {
+  u32x4 c;
+  c.x = pair;
+  c.y = step;
+  c.z = uint32_t(traj & 0xffffffffu);
+  c.w = (tag << 24) | uint32_t((traj >> 32) & 0xffffffu);
+  return c;
+}
+
+// FP64 Box-Muller with accurate libm/ocml log, sqrt, cos, sin.
+__host__ __device__ __forceinline__ void box_muller_f64(uint32_t a, uint32_t b, double& re, double& im) {
+  const double two_m32 = 2.3283064365386963e-10;
+  double u1 = (double(a) + 0.5) * two_m32;
+  double th = double(b) * two_m32;
+  double r = sqrt(-2.0 * log(u1));
+  double ang = 6.283185307179586 * th;
+  re = r * cos(ang);
+  im = r * sin(ang);
+}
+
+// FP32 Box-Muller on the hardware transcendental units: v_log_f32 (log2),
+// v_sqrt_f32, and v_sin_f32 / v_cos_f32, which take revolutions, so the
+// 2 pi factor costs nothing.  u1 lies in (0, 1]; u1 = 1 gives r = 0.
+__device__ __forceinline__ void box_muller_f32(uint32_t a, uint32_t b, float& re, float& im) {
+  const float two_m32 = 2.3283064365386963e-10f;
+  const float two_m33 = 1.1641532182693481e-10f;
+  float u1 = fmaf(float(a), two_m32, two_m33);
+  float th = float(b) * two_m32;
+  float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+  re = r * __builtin_amdgcn_cosf(th);
+  im = r * __builtin_amdgcn_sinf(th);
+}
+
+}  // namespace twmc
+
+#endif

@@ -1,0 +1,138 @@
+"""FP64 HIP kernels vs the CPU oracle, per trajectory (same Philox increments).
+
+Bar: every stored frame of every trajectory within 1e-9 relative (max |diff| /
+max |beta|) after 1000 steps; frame counts, step counts and t_final identical.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import c1_spec, oracle_traj, rel_err, system
+from pincosimulator_amd import Engine
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-9
+
+
+def _compare(spec, ids, precision=64, kernel="generic", **kw):
+    eng = Engine(system(spec), precision=precision, kernel=kernel)
+    res = eng.run(ids, traj=True, final=True, **kw)
+    ok_kw = {k: v for k, v in kw.items() if k in ("t_start", "t_end", "store_initial", "step0")}
+    if "init" in kw:
+        ok_kw["init_mode"] = {"settings": 0, "fixedpoint": 1, "previous": 2}[kw["init"]]
+    if "beta_prev" in kw:
+        ok_kw["beta_prev"] = None
+    frames, finals, tfs = [], [], []
+    for j, t in enumerate(ids):
+        okw = dict(ok_kw)
+        if "beta_prev" in kw:
+            okw["beta_prev"] = kw["beta_prev"][j]
+        fr, bf, tf = oracle.run_trajectory(
+            {**spec, "lattice": oracle.LAT_DENSE if spec.get("lattice") == "matrix" else oracle.LAT_FFT,
+             "E": spec.get("omega", 0.0)}, int(t), **okw)
+        frames.append(fr)
+        finals.append(bf)
+        tfs.append(tf)
+    frames = np.array(frames)
+    eng.close()
+    return res, frames, np.array(finals), tfs
+
+
+def test_chain64_fp64_1000_steps():
+    """1D chain N=64 (configuration 1 parameters), 1000 steps, 16 trajectories."""
+    spec = c1_spec(t_end=5.0)  # 1000 steps of dt = 0.005, frames every 200
+    ids = np.arange(16) + 7
+    res, frames, finals, tfs = _compare(spec, ids)
+    assert res.n_steps == 1000 and res.n_frames == 6
+    assert res.traj.shape == frames.shape
+    assert rel_err(res.traj, frames) < TOL64
+    assert rel_err(res.beta_final, finals) < TOL64
+    assert res.t_final == tfs[0]
+
+
+@pytest.mark.parametrize("nx,ny", [(8, 8), (12, 10), (48, 48), (16, 15), (2, 2)])
+def test_square_fp64(nx, ny):
+    spec = dict(nx=nx, ny=ny, dt=0.002, gamma=1.0, J=1.0, U=0.05, omega=1.0, F=2.0, beta_init=0.0,
+                beta_sigma=0.5, frame_steps=250, seed=99, t_start=0.0, t_end=2.0)
+    ids = np.arange(6) * 1000 + 3
+    res, frames, finals, _ = _compare(spec, ids)
+    assert res.n_steps == 1000
+    assert rel_err(res.traj, frames) < TOL64
+
+
+def test_fp64_64x64_subset():
+    spec = dict(nx=64, ny=64, dt=0.002, gamma=1.0, J=1.0, U=0.05, omega=1.0, F=2.0, beta_init=0.0,
+                beta_sigma=0.5, frame_steps=500, seed=2024, t_start=0.0, t_end=2.0)
+    ids = np.array([0, 1, 16383])
+    res, frames, _, _ = _compare(spec, ids)
+    assert res.n_frames == 3
+    assert rel_err(res.traj, frames) < TOL64
+
+
+def test_noisy_parameters_and_disorder():
+    """Per-trajectory Gaussian/uniform disorder of U, omega, F and beta_init."""
+    rng = np.random.default_rng(5)
+    spec = dict(nx=1, ny=32, dt=0.005, gamma=1.0, J=0.45, frame_steps=100, seed=77, t_start=0.0, t_end=5.0,
+                U={"base": 0.05, "noise": ("gaussian", 0.02 + 0.01j)},
+                omega={"base": rng.normal(size=32) + 0j, "noise": ("uniform", 0.3)},
+                F={"base": 1.0, "noise": ("gaussian", 0.3)},
+                beta_init={"base": 0.1 + 0.2j, "noise": ("gaussian", 0.1 + 0.1j)}, beta_sigma=0.5)
+    ids = np.arange(5) + 100
+    res, frames, _, _ = _compare(spec, ids)
+    assert np.all(np.isfinite(frames))
+    assert rel_err(res.traj, frames) < TOL64
+
+
+def test_lieb_matrix_hopping():
+    """Lieb chain through matrix hopping: h = -i beta.C (TWMCLiebSolver)."""
+    ny = 24
+    C = oracle.lieb_coupling(ny, 4.0, 4.0, False)
+    spec = dict(nx=1, ny=3 * ny, dt=0.0005 * 10, gamma=1.0, J=0.0, U=0.1, omega=0.2 * np.arange(3 * ny) / 72,
+                F=1.0, beta_init=0.0, beta_sigma=0.5, frame_steps=100, seed=7, t_start=0.0, t_end=5.0,
+                lattice="matrix", coupling=C, detuning=2.0, onsite="detuning-E")
+    ids = np.arange(4)
+    res, frames, _, _ = _compare(spec, ids)
+    assert res.n_steps == 1000
+    assert rel_err(res.traj, frames) < TOL64
+
+
+def test_init_modes_and_resume():
+    """FixedPoint start; ReadFromPreviousData continuation with step0 resumes the
+    same Philox stream, so split runs equal one uninterrupted run."""
+    spec = c1_spec(t_end=2.0, frame_steps=100)
+    ids = np.arange(3)
+    res, frames, _, _ = _compare(spec, ids, init="fixedpoint")
+    assert rel_err(res.traj, frames) < TOL64
+    eng = Engine(system(spec), precision=64, kernel="generic")
+    full = eng.run(ids, traj=True, final=True)
+    first = eng.run(ids, t_end=1.0, traj=True, final=True)
+    second = eng.run(ids, t_start=1.0, t_end=2.0, init="previous", beta_prev=first.beta_final,
+                     store_initial=False, step0=first.n_steps, traj=True, final=True)
+    assert first.n_steps + second.n_steps == full.n_steps
+    np.testing.assert_array_equal(second.beta_final, full.beta_final)
+    np.testing.assert_array_equal(second.traj[:, -1], full.traj[:, -1])
+    eng.close()
+
+
+def test_ensemble_sums_match_trajectories():
+    """Engine FP64 ensemble sums equal sums over the stored trajectories."""
+    from pincosimulator_amd.observables import from_trajectories
+    spec = dict(nx=12, ny=10, dt=0.002, gamma=1.0, J=1.0, U=0.05, omega=1.0, F=2.0, beta_init=0.0,
+                beta_sigma=0.5, frame_steps=100, seed=5, t_start=0.0, t_end=1.0)
+    eng = Engine(system(spec), precision=64, kernel="generic")
+    res = eng.run(np.arange(40), traj=True)
+    ref = from_trajectories(res.traj, (12, 10))
+    for k in ("sum_n", "sum_n2", "sum_nk", "sum_beta", "traj_m2", "traj_m4"):
+        assert rel_err(getattr(res, k), ref[k]) < 1e-12, k
+    eng.close()
+
+
+def test_truncated_schedule_matches_reference_loop():
+    """t_end not a multiple of dt_obs: frame count and t_final follow the
+    reference's FP64 time loop exactly."""
+    spec = c1_spec(t_end=1.2345, frame_steps=37, dt=0.0123)
+    res, frames, _, tfs = _compare(spec, np.arange(2))
+    assert res.n_frames == frames.shape[1]
+    assert res.t_final == tfs[0]
+    assert rel_err(res.traj, frames) < TOL64

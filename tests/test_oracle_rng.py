@@ -1,0 +1,83 @@
+"""Pins the oracle's Philox4x32-10 and Box-Muller (CPU only)."""
+import json
+import math
+import os
+
+import numpy as np
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# Known-answer vectors published with Random123 (kat_vectors, philox4x32 10 rounds)
+KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+def test_philox_random123_kat():
+    for ctr, key, out in KAT:
+        assert oracle.philox(ctr, key) == out
+
+
+def test_philox_matches_rocrand_engine():
+    data = json.load(open(os.path.join(HERE, "golden", "philox_rocrand.json")))
+    assert len(data["cases"]) >= 6
+    for c in data["cases"]:
+        assert oracle.philox(c["ctr"], c["key"]) == c["out"], c
+
+
+def test_counter_layout():
+    # site 2p and 2p+1 share one Philox call; tag and trajectory id fill c2/c3
+    seed, traj, tag, step = (0x0123456789ABCDEF, (5 << 32) | 77, 3, 1234)
+    import ctypes as C
+    ctr = (C.c_uint32 * 4)()
+    key = (C.c_uint32 * 2)()
+    oracle.lib().orc_counter(seed, traj, tag, step, 11, ctr, key)
+    assert list(ctr) == [5, 1234, 77, (3 << 24) | 5]
+    assert list(key) == [0x89ABCDEF, 0x01234567]
+    x = oracle.philox(list(ctr), list(key))
+    z_odd = oracle.normal(seed, traj, tag, step, 11)
+    z_even = oracle.normal(seed, traj, tag, step, 10)
+    assert z_odd == oracle.box_muller(x[2], x[3])
+    assert z_even == oracle.box_muller(x[0], x[1])
+
+
+def test_box_muller_known_transforms():
+    # u1 = 1/2 (a = 2^31 - 1/2 rounds to a = 2^31 - 1 + 1/2 ... use exact a = 2^31 - 1: u1 = (2^31-0.5)/2^32)
+    a = 2 ** 31 - 1
+    u1 = (a + 0.5) / 2 ** 32
+    r = math.sqrt(-2 * math.log(u1))
+    z = oracle.box_muller(a, 0)
+    assert abs(z.real - r) < 1e-15 and z.imag == 0.0
+    z = oracle.box_muller(a, 2 ** 30)  # theta = 1/4 revolution
+    assert abs(z.real) < 1e-15 and abs(z.imag - r) < 1e-15
+    z = oracle.box_muller(a, 2 ** 31)  # theta = 1/2
+    assert abs(z.real + r) < 1e-15 and abs(z.imag) < 1e-15
+    # extremes stay finite: u1 in (0, 1)
+    assert np.isfinite(oracle.box_muller(0, 123).real)
+    assert abs(oracle.box_muller(2 ** 32 - 1, 0)) < 1e-4
+
+
+def test_normal_moments():
+    n = 40000
+    z = np.array([oracle.normal(99, 3, 0, 17, i) for i in range(n)])
+    re, im = z.real, z.imag
+    tol = 5 / math.sqrt(n)
+    assert abs(re.mean()) < tol and abs(im.mean()) < tol
+    assert abs(re.var() - 1) < 5 * math.sqrt(2 / n) and abs(im.var() - 1) < 5 * math.sqrt(2 / n)
+    assert abs(np.mean(re * im)) < tol
+    kurt = np.mean(re ** 4) / np.mean(re ** 2) ** 2
+    assert abs(kurt - 3) < 0.15
+    # independence across steps and trajectories
+    z2 = np.array([oracle.normal(99, 4, 0, 17, i) for i in range(2000)])
+    assert abs(np.mean(z[:2000].real * z2.real)) < 5 / math.sqrt(2000)
+
+
+def test_uniform_range():
+    u = np.array([oracle.uniform(1, 2, 2, 0, i) for i in range(4000)])
+    assert u.min() > 0 and u.max() < 1
+    assert abs(u.mean() - 0.5) < 0.02
