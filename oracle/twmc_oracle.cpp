@@ -47,8 +47,8 @@ inline void philox10(const uint32_t c_in[4], const uint32_t k_in[2], uint32_t ou
 
 inline void make_counter(uint64_t seed, uint64_t traj, uint32_t tag, uint32_t step,
                          uint64_t site, uint32_t ctr[4], uint32_t key[2]) {
-  ctr[0] = uint32_t(site >> 1);
-  ctr[1] = step;
+  ctr[0] = step;
+  ctr[1] = uint32_t(site >> 1);
   ctr[2] = uint32_t(traj & 0xffffffffu);
   ctr[3] = (tag << 24) | uint32_t((traj >> 32) & 0xffffffu);
   key[0] = uint32_t(seed & 0xffffffffu);

@@ -52,7 +52,7 @@ enum { ORC_INIT_SETTINGS = 0, ORC_INIT_FIXEDPOINT = 1, ORC_INIT_PREVIOUS = 2 };
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
 /* Counter layout shared with the HIP kernels:
- *   ctr = { site/2, step, traj & 0xffffffff, (tag << 24) | ((traj >> 32) & 0xffffff) }
+ *   ctr = { step, site/2, traj & 0xffffffff, (tag << 24) | ((traj >> 32) & 0xffffff) }
  *   key = { seed & 0xffffffff, seed >> 32 }
  * Site 2p uses output words (x0, x1), site 2p+1 uses (x2, x3). */
 void orc_counter(uint64_t seed, uint64_t traj, uint32_t tag, uint32_t step,

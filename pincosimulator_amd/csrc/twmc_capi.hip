@@ -197,7 +197,13 @@ void plan_segments(Ctx* c) {
 
 int choose_kernel(Ctx* c) {
   const twmc_params& p = c->p;
-  bool fused_ok = fused64_supported(p.nx, p.ny, p.hopping, c->precision, c->noisy_params ? 1 : 0);
+  // the register-resident kernel takes site-uniform, trajectory-shared parameters
+  bool uniform = true;
+  for (int k = 0; k < 3; ++k)
+    for (int i = 1; i < c->n && uniform; ++i)
+      uniform = c->base[k][2 * i] == c->base[k][0] && c->base[k][2 * i + 1] == c->base[k][1];
+  bool fused_ok = fused64_supported(p.nx, p.ny, p.hopping, c->precision,
+                                    (c->noisy_params || !uniform) ? 1 : 0);
   size_t lds = generic_lds_bytes(p.nx, p.ny, c->precision);
   bool generic_ok = generic_pairs_per_thread(c->n, 256) > 0 && lds <= 160 * 1024 &&
                     (p.hopping == TWMC_HOP_MATRIX || (c->px.npass >= 0 && c->py.npass >= 0));
@@ -205,7 +211,7 @@ int choose_kernel(Ctx* c) {
   if (k == TWMC_KERNEL_AUTO) k = fused_ok ? TWMC_KERNEL_FUSED64 : TWMC_KERNEL_GENERIC;
   if (k == TWMC_KERNEL_FUSED64 && !fused_ok)
     return fail(c, TWMC_ERR_UNSUPPORTED,
-                "fused 64x64 kernel needs nx = ny = 64, FFT hopping, FP32 and no per-trajectory disorder");
+                "fused 64x64 kernel needs nx = ny = 64, FFT hopping, FP32 and site-uniform, disorder-free parameters");
   if (k == TWMC_KERNEL_GENERIC && !generic_ok)
     return fail(c, TWMC_ERR_UNSUPPORTED, "lattice too large for the LDS-resident kernel");
   if (k == TWMC_KERNEL_MFMA)

@@ -2,9 +2,11 @@
 //
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed by the 64-bit run
 // seed, with the counter
-//     { site/2, step, traj & 0xffffffff, (tag << 24) | ((traj >> 32) & 0xffffff) }
+//     { step, site/2, traj & 0xffffffff, (tag << 24) | ((traj >> 32) & 0xffffff) }
 // so every increment is a pure function of (seed, trajectory id, step, site):
 // results do not depend on batch composition, launch shape or resume point.
+// The words multiplied in the first round (step and trajectory) are uniform
+// across a wavefront, so that round runs on the scalar unit.
 // Site 2p takes output words (x0, x1), site 2p+1 takes (x2, x3); each pair is
 // turned into one complex N(0,1) sample (Re, Im independent) by Box-Muller:
 //     u1 = (a + 1/2) 2^-32,  theta = b 2^-32 (revolutions),
@@ -64,8 +66,8 @@ __host__ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, ui
 
 __host__ __device__ __forceinline__ u32x4 make_counter(uint64_t traj, uint32_t tag, uint32_t step, uint32_t pair) {
   u32x4 c;
-  c.x = pair;
-  c.y = step;
+  c.x = step;
+  c.y = pair;
   c.z = uint32_t(traj & 0xffffffffu);
   c.w = (tag << 24) | uint32_t((traj >> 32) & 0xffffffu);
   return c;
