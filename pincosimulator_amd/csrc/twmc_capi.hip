@@ -68,7 +68,8 @@ struct Ctx {
   int noise[5] = {0, 0, 0, 0, 0};
   // device-resident constants
   DevBuf d_base[5], d_amp[5], d_twx, d_twy, d_kcx, d_kcy, d_cscp, d_cscr, d_cscv, d_perm_site,
-      d_perm_k;
+      d_perm_k, d_ahi, d_alo;
+  int np = 0;  // padded site count of the MFMA kernel
   // per-batch buffers
   DevBuf d_ids, d_beta, d_rsl, d_U, d_F, d_partials, d_obs, d_m2, d_m4, d_traj, d_prev;
   int csc_nnz = 0;
@@ -207,15 +208,18 @@ int choose_kernel(Ctx* c) {
   size_t lds = generic_lds_bytes(p.nx, p.ny, c->precision);
   bool generic_ok = generic_pairs_per_thread(c->n, 256) > 0 && lds <= 160 * 1024 &&
                     (p.hopping == TWMC_HOP_MATRIX || (c->px.npass >= 0 && c->py.npass >= 0));
+  bool mfma_ok = mfma_supported(c->n, p.hopping, c->precision, c->noisy_params ? 1 : 0);
   int k = p.kernel;
-  if (k == TWMC_KERNEL_AUTO) k = fused_ok ? TWMC_KERNEL_FUSED64 : TWMC_KERNEL_GENERIC;
+  if (k == TWMC_KERNEL_AUTO)
+    k = fused_ok ? TWMC_KERNEL_FUSED64 : mfma_ok ? TWMC_KERNEL_MFMA : TWMC_KERNEL_GENERIC;
   if (k == TWMC_KERNEL_FUSED64 && !fused_ok)
     return fail(c, TWMC_ERR_UNSUPPORTED,
                 "fused 64x64 kernel needs nx = ny = 64, FFT hopping, FP32 and site-uniform, disorder-free parameters");
   if (k == TWMC_KERNEL_GENERIC && !generic_ok)
     return fail(c, TWMC_ERR_UNSUPPORTED, "lattice too large for the LDS-resident kernel");
-  if (k == TWMC_KERNEL_MFMA)
-    return fail(c, TWMC_ERR_UNSUPPORTED, "MFMA matrix-hopping kernel is not available in this build");
+  if (k == TWMC_KERNEL_MFMA && !mfma_ok)
+    return fail(c, TWMC_ERR_UNSUPPORTED,
+                "MFMA kernel needs matrix hopping, FP32, at most 512 sites and no per-trajectory disorder");
   c->kernel = k;
   return TWMC_OK;
 }
@@ -360,6 +364,20 @@ int twmc_create(const twmc_params* p, int device, void** ctx_out) {
     if (ee == hipSuccess) up(c->d_cscv, val, false);
     if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "upload coupling");
   }
+  if (fail_rc == TWMC_OK && c->kernel == TWMC_KERNEL_MFMA) {
+    // A = C^T, zero-padded to np x np, split into BF16 high and low parts
+    c->np = mfma_padded(c->n);
+    const int np = c->np;
+    std::vector<double> a((size_t)np * np, 0.0);
+    for (int i = 0; i < c->n; ++i)
+      for (int j = 0; j < c->n; ++j) a[(size_t)j * np + i] = p->coupling[(size_t)i * c->n + j];
+    std::vector<uint16_t> hi((size_t)np * np), lo((size_t)np * np);
+    mfma_split(a.data(), np, hi.data(), lo.data());
+    hipError_t ee = upload(c->d_ahi, hi.data(), hi.size() * 2, s);
+    if (ee == hipSuccess) ee = upload(c->d_alo, lo.data(), lo.size() * 2, s);
+    if (ee == hipSuccess) ee = hipStreamSynchronize(s);
+    if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "upload MFMA operand");
+  }
   if (fail_rc == TWMC_OK && c->kernel == TWMC_KERNEL_FUSED64) {
     std::vector<int> ps(4096), pk(4096);
     fused64_perms(ps.data(), pk.data());
@@ -394,7 +412,7 @@ void twmc_destroy(void* ctx) {
     (void)hipEventDestroy(pr.second);
   }
   DevBuf* bufs[] = {&c->d_twx, &c->d_twy, &c->d_kcx, &c->d_kcy, &c->d_cscp, &c->d_cscr,
-                    &c->d_cscv, &c->d_perm_site, &c->d_perm_k, &c->d_ids, &c->d_beta, &c->d_rsl,
+                    &c->d_cscv, &c->d_perm_site, &c->d_perm_k, &c->d_ahi, &c->d_alo, &c->d_ids, &c->d_beta, &c->d_rsl,
                     &c->d_U, &c->d_F, &c->d_partials, &c->d_obs, &c->d_m2, &c->d_m4, &c->d_traj,
                     &c->d_prev};
   for (DevBuf* b : bufs) b->release();
@@ -431,17 +449,23 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
   int cus = 0;
   HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
   int per_cu = 1;
+  size_t units = B;  // work items of the persistent grid
   if (c->kernel == TWMC_KERNEL_FUSED64) {
     c->block = 64;
     c->lds = 0;
     HIPCHK(c, fused64_occupancy(&per_cu));
+  } else if (c->kernel == TWMC_KERNEL_MFMA) {
+    c->block = 256;
+    c->lds = mfma_lds_bytes(c->np);
+    units = (B + 31) / 32;
+    HIPCHK(c, mfma_occupancy(c->np, &per_cu));
   } else {
     c->block = 256;
     c->lds = generic_lds_bytes(c->p.nx, c->p.ny, c->precision);
     HIPCHK(c, generic_occupancy(c->precision, n, c->block, c->lds, &per_cu));
   }
   if (per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");
-  c->grid = (int)std::min<size_t>(B, (size_t)per_cu * cus);
+  c->grid = (int)std::min<size_t>(units, (size_t)per_cu * cus);
 
   size_t sets = c->noisy_params ? B : 1;
   HIPCHK(c, upload(c->d_ids, d->traj_ids, B * sizeof(uint64_t), s));
@@ -552,6 +576,9 @@ int twmc_batch_advance(void* ctx, uint64_t n_segments) {
       a.traj_m2 = static_cast<double*>(c->d_m2.p);
       a.traj_m4 = static_cast<double*>(c->d_m4.p);
       a.traj_out = c->d_traj.p;
+      a.a_hi = c->d_ahi.p;
+      a.a_lo = c->d_alo.p;
+      a.np = c->np;
       if (a.n_steps > 0 || a.capture) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (c->timing) {
@@ -561,6 +588,8 @@ int twmc_batch_advance(void* ctx, uint64_t n_segments) {
         }
         if (c->kernel == TWMC_KERNEL_FUSED64) {
           HIPCHK(c, launch_fused64(a, c->grid, s));
+        } else if (c->kernel == TWMC_KERNEL_MFMA) {
+          HIPCHK(c, launch_mfma(a, c->grid, s));
         } else {
           HIPCHK(c, launch_generic(a, c->precision, c->grid, c->block, c->lds, s));
         }

@@ -52,6 +52,10 @@ struct StepArgs {
   double* traj_m2;           // [n_traj][n_frames]
   double* traj_m4;           // [n_traj][n_frames]
   void* traj_out;            // [n_traj][n_frames][n] complex (T)
+  // MFMA matrix hopping: A = C^T split into BF16 high and low parts, [np][np]
+  const void* a_hi;
+  const void* a_lo;
+  int np;                    // n padded to a multiple of 32
 };
 
 struct InitArgs {
@@ -94,6 +98,15 @@ bool fused64_supported(int nx, int ny, int hopping, int precision, int param_str
 hipError_t launch_fused64(const StepArgs& a, int grid, hipStream_t s);
 hipError_t fused64_occupancy(int* blocks_per_cu);
 void fused64_perms(int* perm_site, int* perm_k);  // [4096] each
+
+// matrix hopping on MFMA (FP32 state, BF16 x 3 products, FP32 accumulation)
+bool mfma_supported(int n, int hopping, int precision, int param_stride);
+int mfma_padded(int n);
+size_t mfma_lds_bytes(int np);
+hipError_t launch_mfma(const StepArgs& a, int grid, hipStream_t s);
+hipError_t mfma_occupancy(int np, int* blocks_per_cu);
+// splits A (row-major np x np, FP64) into BF16 hi and lo parts (raw 16-bit)
+void mfma_split(const double* a, int np, uint16_t* hi, uint16_t* lo);
 
 }  // namespace twmc
 

@@ -1,0 +1,326 @@
+// twmc_mfma.hip -- matrix-hopping TWMC step on the matrix cores.
+//
+// For general (disordered, Lieb, long-range) hopping the reference computes
+// h = -i (beta . C) per trajectory with a sparse product
+// (src/TWMC/Solvers/TWMCLiebSolver.cpp:259-267).  Here the hopping of a block
+// of 32 trajectories is one GEMM per step,
+//     Y = C^T [Re beta_1 .. Re beta_32 | Im beta_1 .. Im beta_32]   (N x 64),
+// on v_mfma_f32_32x32x16_bf16 with FP32 accumulation.  FP32 operands are
+// split as x = x_hi + x_lo (both BF16) and the product is
+//     A_hi X_hi + A_hi X_lo + A_lo X_hi,
+// which keeps about 16 significant bits (the dropped A_lo X_lo term and the
+// BF16 rounding of the low parts are ~2^-16 relative).
+//
+// One workgroup (4 waves) owns 32 trajectories for a whole segment.  Wave w
+// owns TPW row tiles of 32 sites; its accumulators hold Y for those sites and
+// both column tiles (Re, Im), so each lane has Re and Im of the same
+// (site, trajectory) and the Euler update runs in the GEMM epilogue, in
+// registers, on the FP32 state that never leaves the lane.  Each step the
+// state is re-staged in LDS as the BF16 hi/lo B operand; A = C^T (hi/lo) is
+// read from L2 (it is shared by every workgroup).
+//
+// MFMA lane layout (gfx950, checked on the device): A lane l holds row l%32,
+// k = (l/32)*8 + 0..7; B lane l holds column l%32, same k; result register r
+// of lane l is row (r/4)*8 + (l/32)*4 + r%4, column l%32.
+#include <cstring>
+
+#include "twmc_cplx.h"
+#include "twmc_internal.h"
+#include "twmc_rng.h"
+
+namespace twmc {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kTB = 32;  // trajectories per workgroup
+
+__device__ __forceinline__ uint16_t bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf16_float(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+
+template <int TPW, bool CAPTURE_ONLY>
+__global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int np = g.np, n = g.n;
+  const int ldk = np + 8;  // padded row: conflict-free 16-byte column reads
+  uint16_t* Xhi = reinterpret_cast<uint16_t*>(smem);
+  uint16_t* Xlo = Xhi + 64 * ldk;
+  cpx<float>* Prsl = reinterpret_cast<cpx<float>*>(Xlo + 64 * ldk);
+  cpx<float>* PU = Prsl + np;
+  cpx<float>* PF = PU + np;
+  double* red = reinterpret_cast<double*>(PF + np);  // [4 waves][32 traj][2]
+
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = lane & 31, half = lane >> 5;
+  const int tiles = np / 32;  // wave w owns row tiles w*TPW .. w*TPW+TPW-1 that are < tiles
+  for (int i = tid; i < np; i += blockDim.x) {
+    const bool in = i < n;
+    Prsl[i] = in ? reinterpret_cast<const cpx<float>*>(g.rsl)[i] : mk(0.f, 0.f);
+    PU[i] = in ? reinterpret_cast<const cpx<float>*>(g.U)[i] : mk(0.f, 0.f);
+    PF[i] = in ? reinterpret_cast<const cpx<float>*>(g.F)[i] : mk(0.f, 0.f);
+  }
+  __syncthreads();
+
+  const float dt = float(g.dt), sig = float(g.dt4);
+  const uint16_t* Ahi = reinterpret_cast<const uint16_t*>(g.a_hi);
+  const uint16_t* Alo = reinterpret_cast<const uint16_t*>(g.a_lo);
+  double* part = g.partials + (size_t)blockIdx.x * kQ * n;
+  const int nblk = (g.n_traj + kTB - 1) / kTB;
+
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int j = blk * kTB + col;
+    const bool valid = j < g.n_traj;
+    const uint64_t traj = valid ? g.traj_ids[j] : 0;
+    cpx<float>* gb = reinterpret_cast<cpx<float>*>(g.beta) + (size_t)(valid ? j : 0) * n;
+
+    // FP32 state in the MFMA result layout: [tile][register]
+    float sre[TPW][16], sim[TPW][16];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = (wave * TPW + t) * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+        cpx<float> v = (valid && wave * TPW + t < tiles && i < n) ? gb[i] : mk(0.f, 0.f);
+        sre[t][r] = v.re;
+        sim[t][r] = v.im;
+      }
+
+    const int iters = CAPTURE_ONLY ? 1 : g.n_steps;
+    for (int s = 0; s < iters; ++s) {
+      // stage the state as the BF16 hi/lo B operand: column col (Re) and 32+col (Im)
+#pragma unroll
+      for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (wave * TPW + t >= tiles) continue;
+          const int i0 = (wave * TPW + t) * 32 + q * 8 + half * 4;
+          uint16_t rh[4], rl[4], ih[4], il[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const float xr = sre[t][4 * q + e], xi = sim[t][4 * q + e];
+            rh[e] = bf16_bits(xr);
+            rl[e] = bf16_bits(xr - bf16_float(rh[e]));
+            ih[e] = bf16_bits(xi);
+            il[e] = bf16_bits(xi - bf16_float(ih[e]));
+          }
+          auto pack = [](const uint16_t* v) {
+            return make_uint2(uint32_t(v[0]) | (uint32_t(v[1]) << 16), uint32_t(v[2]) | (uint32_t(v[3]) << 16));
+          };
+          *reinterpret_cast<uint2*>(&Xhi[col * ldk + i0]) = pack(rh);
+          *reinterpret_cast<uint2*>(&Xlo[col * ldk + i0]) = pack(rl);
+          *reinterpret_cast<uint2*>(&Xhi[(32 + col) * ldk + i0]) = pack(ih);
+          *reinterpret_cast<uint2*>(&Xlo[(32 + col) * ldk + i0]) = pack(il);
+        }
+      __syncthreads();
+
+      if (s == 0 && g.capture) {
+        double m2 = 0.0, m4 = 0.0;
+        cpx<float>* out = (g.dump && valid)
+                              ? reinterpret_cast<cpx<float>*>(g.traj_out) + ((size_t)j * g.n_frames + g.frame) * n
+                              : nullptr;
+#pragma unroll
+        for (int t = 0; t < TPW; ++t)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int i = (wave * TPW + t) * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+            const bool use = valid && wave * TPW + t < tiles && i < n;
+            const double xr = use ? double(sre[t][r]) : 0.0, xi = use ? double(sim[t][r]) : 0.0;
+            const double nn = xr * xr + xi * xi;
+            m2 += nn;
+            m4 += nn * nn;
+            if (out && use) out[i] = mk(sre[t][r], sim[t][r]);
+            // per-site sums over the block's 32 trajectories: the 32 lanes of
+            // this half-wave hold the same site for different trajectories
+            double a = nn, b = nn * nn, c = xr, d = xi;
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) {
+              a += __shfl_xor(a, o, 64);
+              b += __shfl_xor(b, o, 64);
+              c += __shfl_xor(c, o, 64);
+              d += __shfl_xor(d, o, 64);
+            }
+            if (col == 0 && wave * TPW + t < tiles && i < n) {
+              part[i] += a;
+              part[n + i] += b;
+              part[3 * n + i] += c;
+              part[4 * n + i] += d;
+            }
+          }
+        // lanes col and col+32 hold the same trajectory
+        m2 += __shfl_xor(m2, 32, 64);
+        m4 += __shfl_xor(m4, 32, 64);
+        if (half == 0) {
+          red[(wave * 32 + col) * 2] = m2;
+          red[(wave * 32 + col) * 2 + 1] = m4;
+        }
+        __syncthreads();
+        if (tid < 32 && blk * kTB + tid < g.n_traj) {
+          double a2 = 0.0, a4 = 0.0;
+          for (int w = 0; w < 4; ++w) {
+            a2 += red[(w * 32 + tid) * 2];
+            a4 += red[(w * 32 + tid) * 2 + 1];
+          }
+          const size_t o = (size_t)(blk * kTB + tid) * g.n_frames + g.frame;
+          g.traj_m2[o] = a2 / double(n);
+          g.traj_m4[o] = a4 / double(n);
+        }
+        __syncthreads();
+      }
+      if (CAPTURE_ONLY) break;
+
+      f32x16 are[TPW], aim[TPW];
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          are[t][r] = 0.f;
+          aim[t][r] = 0.f;
+        }
+      }
+      for (int kb = 0; kb < np; kb += 16) {
+        const int ko = kb + half * 8;
+        const bf16x8 brh = *reinterpret_cast<const bf16x8*>(&Xhi[col * ldk + ko]);
+        const bf16x8 brl = *reinterpret_cast<const bf16x8*>(&Xlo[col * ldk + ko]);
+        const bf16x8 bih = *reinterpret_cast<const bf16x8*>(&Xhi[(32 + col) * ldk + ko]);
+        const bf16x8 bil = *reinterpret_cast<const bf16x8*>(&Xlo[(32 + col) * ldk + ko]);
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+          if (wave * TPW + t >= tiles) continue;
+          const size_t arow = (size_t)((wave * TPW + t) * 32 + col) * np + ko;
+          const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&Ahi[arow]);
+          const bf16x8 al = *reinterpret_cast<const bf16x8*>(&Alo[arow]);
+          are[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, brh, are[t], 0, 0, 0);
+          aim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bih, aim[t], 0, 0, 0);
+          are[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, brl, are[t], 0, 0, 0);
+          aim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bil, aim[t], 0, 0, 0);
+          are[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, brh, are[t], 0, 0, 0);
+          aim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bih, aim[t], 0, 0, 0);
+        }
+      }
+      __syncthreads();  // the next step overwrites the B operand
+
+      // epilogue: h = -i Y; drift, increment and update (TWMCLiebSolver.cpp:259-267)
+      const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
+      // site-pair base of this lane; opaque to the compiler so the per-pair
+      // second Philox round is not hoisted out of the step loop
+      uint32_t pbase = (uint32_t)(wave * TPW * 16 + half * 2);
+      asm volatile("" : "+v"(pbase));
+#pragma unroll
+      for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int pr = 0; pr < 2; ++pr) {
+            if (wave * TPW + t >= tiles) continue;
+            const int i0 = (wave * TPW + t) * 32 + q * 8 + half * 4 + 2 * pr;  // even site
+            const u32x4 w = philox4x32_10(make_counter(traj, TAG_WIENER, step, pbase + (uint32_t)(t * 16 + q * 4 + pr)),
+                                          g.key0, g.key1);
+            float xr[2], xi[2];
+            box_muller_f32(w.x, w.y, xr[0], xi[0]);
+            box_muller_f32(w.z, w.w, xr[1], xi[1]);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const int r = 4 * q + 2 * pr + h;
+              const int i = i0 + h;
+              if (i < n) {
+                const float br = sre[t][r], bi = sim[t][r];
+                const cpx<float> rs = Prsl[i], u = PU[i], f = PF[i];
+                const float qq = br * br + bi * bi - 1.0f;
+                const float cr = rs.re - u.im * qq, ci = rs.im + u.re * qq;
+                const float dr = cr * br - ci * bi + aim[t][r] - f.im;  // h_re = Im Y, i F = (-F.im, F.re)
+                const float di = cr * bi + ci * br - are[t][r] + f.re;  // h_im = -Re Y
+                sre[t][r] = br + dt * dr + sig * xr[h];
+                sim[t][r] = bi + dt * di + sig * xi[h];
+              }
+            }
+          }
+    }
+
+    if (!CAPTURE_ONLY && valid) {
+#pragma unroll
+      for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int i = (wave * TPW + t) * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+          if (wave * TPW + t < tiles && i < n) gb[i] = mk(sre[t][r], sim[t][r]);
+        }
+    }
+  }
+}
+
+template <int TPW>
+hipError_t launch_t(const StepArgs& a, int grid, size_t lds, hipStream_t s) {
+  const void* f = a.n_steps > 0 ? (const void*)mfma_step_kernel<TPW, false> : (const void*)mfma_step_kernel<TPW, true>;
+  hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  if (a.n_steps > 0)
+    hipLaunchKernelGGL((mfma_step_kernel<TPW, false>), dim3(grid), dim3(256), lds, s, a);
+  else
+    hipLaunchKernelGGL((mfma_step_kernel<TPW, true>), dim3(grid), dim3(256), lds, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int mfma_padded(int n) { return (n + 31) / 32 * 32; }
+
+size_t mfma_lds_bytes(int np) {
+  return (size_t)2 * 64 * (np + 8) * 2 + (size_t)3 * np * 8 + 4 * 32 * 2 * 8;
+}
+
+bool mfma_supported(int n, int hopping, int precision, int param_stride) {
+  int np = mfma_padded(n);
+  return hopping == 1 && precision == 32 && param_stride == 0 && np / 32 <= 16 && mfma_lds_bytes(np) <= 160 * 1024;
+}
+
+hipError_t launch_mfma(const StepArgs& a, int grid, hipStream_t s) {
+  const int tiles = a.np / 32;
+  const int tpw = (tiles + 3) / 4;
+  const size_t lds = mfma_lds_bytes(a.np);
+  switch (tpw) {
+    case 1: return launch_t<1>(a, grid, lds, s);
+    case 2: return launch_t<2>(a, grid, lds, s);
+    case 3: return launch_t<3>(a, grid, lds, s);
+    default: return launch_t<4>(a, grid, lds, s);
+  }
+}
+
+hipError_t mfma_occupancy(int np, int* blocks_per_cu) {
+  const size_t lds = mfma_lds_bytes(np);
+  const int tpw = (np / 32 + 3) / 4;
+  const void* f = tpw == 1 ? (const void*)mfma_step_kernel<1, false>
+                  : tpw == 2 ? (const void*)mfma_step_kernel<2, false>
+                  : tpw == 3 ? (const void*)mfma_step_kernel<3, false>
+                             : (const void*)mfma_step_kernel<4, false>;
+  hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 256, lds);
+}
+
+void mfma_split(const double* a, int np, uint16_t* hi, uint16_t* lo) {
+  auto rne = [](float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+  };
+  auto back = [](uint16_t b) {
+    uint32_t u = uint32_t(b) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+  };
+  for (size_t k = 0; k < (size_t)np * np; ++k) {
+    float f = (float)a[k];
+    hi[k] = rne(f);
+    lo[k] = rne(f - back(hi[k]));
+  }
+}
+
+}  // namespace twmc
