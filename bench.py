@@ -25,21 +25,51 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "trajectory·site-steps/s (2D 64×64, 1 GPU); time-to-steady-state plain vs warm"
+PEAK_FP32 = 157.3e12           # MI355X FP32 vector peak (FLOP/s)
+PEAK_BF16 = 2.5e15             # MI355X dense BF16 MFMA peak (FLOP/s)
+
+# square64: the headline workload (default).  lieb384: the disordered Lieb
+# chain with matrix hopping on MFMA, measured with --workload lieb384.
 N_TRAJ = 16384
 FRAME_STEPS = 500
 FLOPS_PER_SITE_STEP = 150.0    # 2 FFTs x 5 N log2 N / N + 30 elementwise, N = 4096
-PEAK_FP32 = 157.3e12           # MI355X FP32 vector peak (FLOP/s)
 SPEC = dict(nx=64, ny=64, dt=0.002, gamma=1.0, J=1.0, U=0.05, omega=1.0, F=2.0, beta_init=0.0,
             beta_sigma=0.5, frame_steps=FRAME_STEPS, seed=20260101, t_start=0.0)
+LIEB_CELLS, LIEB_TRAJ, LIEB_FRAME = 128, 8192, 200
+
+
+def lieb_spec():
+    from pincosimulator_amd.lattices import lieb_coupling, onsite_disorder
+    n = 3 * LIEB_CELLS
+    C = lieb_coupling(LIEB_CELLS, 1.0, 1.0, pbc=False, sigma=0.1, seed=7)
+    E = onsite_disorder(n, 0.2, 7)
+    return dict(nx=1, ny=n, dt=0.005, gamma=1.0, J=0.0, U=0.1, F=1.5, omega=E, beta_init=0.0,
+                beta_sigma=0.5, frame_steps=LIEB_FRAME, seed=20260102, t_start=0.0, lattice="matrix",
+                coupling=C, detuning=0.0, onsite="detuning-E")
+
+
+WORKLOAD_TEXT = {
+    "square64": "2D 64x64 PBC uniform-hopping TWMC, J=1 U=0.05 omega=1 F=2 gamma=1 dt=0.002, "
+                "16384 trajectories, vacuum Wigner start; one step = one 500-step frame "
+                "incl. FP64 ensemble reduction",
+    "lieb384": "1D Lieb chain 128 cells x 3 = 384 sites OBC, J=1 with bond disorder 0.1 and on-site "
+               "disorder 0.2 (one realisation, seed 7), U=0.1 F=1.5 gamma=1 dt=0.005, 8192 trajectories, "
+               "hopping as a dense 384x384 BF16x3 MFMA GEMM; one step = one 200-step frame",
+}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(threads, target_s=12.0):
+def cpu_baseline(threads, spec=None, target_s=12.0):
     import oracle
-    p = dict(SPEC, lattice=oracle.LAT_FFT)
+    spec = SPEC if spec is None else spec
+    if spec.get("lattice") == "matrix":
+        p = dict(spec, lattice=oracle.LAT_DENSE, E=spec["omega"])
+    else:
+        p = dict(spec, lattice=oracle.LAT_FFT)
+    n_sites = spec["nx"] * spec["ny"]
     # pilot to size the sample to ~target_s of wall time
     n_traj_p, steps_p = threads, 5
     t = oracle.time_batch(p, n_traj_p, threads, steps_p, faithful=True)
@@ -47,12 +77,13 @@ def cpu_baseline(threads, target_s=12.0):
     n_traj = 4 * threads
     steps = max(5, int(target_s * rate / n_traj))
     t = oracle.time_batch(p, n_traj, threads, steps, faithful=True)
-    value = n_traj * 4096 * steps / t
+    value = n_traj * n_sites * steps / t
+    kind = "sparse coupling product" if spec.get("lattice") == "matrix" else "FFT hopping"
     return {
         "value": value, "unit": "trajectory·site-steps/s", "cores": threads, "kind": "port",
-        "sample": f"{n_traj} trajectories x {steps} steps of the same 64x64 system "
-                  f"({t:.1f} s wall), FP64, reference per-step structure with std::mt19937 noise, "
-                  f"one trajectory per task on {threads} std::threads",
+        "sample": f"{n_traj} trajectories x {steps} steps of the same {n_sites}-site system "
+                  f"({t:.1f} s wall), FP64, reference per-step structure ({kind}) with std::mt19937 "
+                  f"noise, one trajectory per task on {threads} std::threads",
     }
 
 
@@ -62,6 +93,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="square64", choices=["square64", "lieb384"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 256 MiB device-to-device copy (torch) whose HBM counters "
                          "calibrate FETCH_SIZE / WRITE_SIZE in a --pmc profile")
@@ -86,9 +118,14 @@ def main():
         torch.cuda.synchronize()
         log("bench: calibration copy of 256 MiB done")
         del a, b
-    spec = dict(SPEC, t_end=(W + K) * FRAME_STEPS * SPEC["dt"])
+    if args.workload == "lieb384":
+        base, n_traj, fsteps = lieb_spec(), LIEB_TRAJ, LIEB_FRAME
+    else:
+        base, n_traj, fsteps = SPEC, N_TRAJ, FRAME_STEPS
+    n_sites = base["nx"] * base["ny"]
+    spec = dict(base, t_end=(W + K) * fsteps * base["dt"])
     eng = Engine(System(**spec), precision=32)
-    info = eng.begin(np.arange(N_TRAJ, dtype=np.uint64))
+    info = eng.begin(np.arange(n_traj, dtype=np.uint64))
     assert info["n_segments"] >= W + K, info
     log(f"bench: kernel={info['kernel']} segments={info['n_segments']} warmup={W} steps={K}")
     eng.advance(W)
@@ -107,17 +144,25 @@ def main():
         sys.exit("non-finite ensemble sums")
 
     wall = t1 - t0
-    site_steps = float(N_TRAJ) * 4096 * FRAME_STEPS * K
+    site_steps = float(n_traj) * n_sites * fsteps * K
     value = site_steps / wall
     avg_ms = st["step_kernel_ms"] / max(st["step_launches"], 1)
-    flops_per_launch = float(N_TRAJ) * 4096 * FRAME_STEPS * FLOPS_PER_SITE_STEP
+    if args.workload == "lieb384":
+        # executed MFMA work: 3 BF16 products of the real N x N hopping on Re and Im
+        flops_site = 3 * 4.0 * n_sites
+        peak, bound, kname = PEAK_BF16, "mfma", "mfma_step_kernel"
+    else:
+        flops_site = FLOPS_PER_SITE_STEP
+        peak, bound, kname = PEAK_FP32, "valu", "fused64_step_kernel"
+    flops_per_launch = float(n_traj) * n_sites * fsteps * flops_site
     achieved = flops_per_launch / (avg_ms * 1e-3)
     log(f"bench: wall {wall:.3f} s, {value:.4e} site-steps/s, step kernel avg {avg_ms:.2f} ms "
         f"over {st['step_launches']} launches, reduce avg "
         f"{st['reduce_kernel_ms'] / max(st['reduce_launches'], 1):.3f} ms, grid {st['grid']}")
 
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "fused64_64x64_16384traj_pmc.json")
+    pmc = os.path.join(ROOT, "profiles", "fused64_64x64_16384traj_pmc.json" if args.workload == "square64"
+                       else "mfma_lieb384_8192traj_pmc.json")
     if os.path.exists(pmc):
         try:
             traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
@@ -129,17 +174,16 @@ def main():
         "steps": K, "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {
-            "workload": "2D 64x64 PBC uniform-hopping TWMC, J=1 U=0.05 omega=1 F=2 gamma=1 dt=0.002, "
-                        "16384 trajectories, vacuum Wigner start; one step = one 500-step frame "
-                        "incl. FP64 ensemble reduction",
-            "lattice": "64x64", "trajectories": N_TRAJ, "euler_steps_per_step": FRAME_STEPS,
+            "workload": WORKLOAD_TEXT[args.workload],
+            "lattice": "64x64" if args.workload == "square64" else "Lieb 128x3 OBC",
+            "trajectories": n_traj, "euler_steps_per_step": fsteps,
             "state": "complex64 in registers/LDS, FP64 ensemble sums", "kernel": info["kernel"],
             "grid": st["grid"], "block": st["block"],
         },
         "roofline": {
-            "bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32, "traffic": traffic,
-            "kernel": "fused64_step_kernel", "avg_launch_ms": avg_ms,
+            "bound": bound, "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": traffic,
+            "kernel": kname, "avg_launch_ms": avg_ms,
             "flops_per_launch": flops_per_launch,
         },
     }
@@ -147,7 +191,7 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"bench: CPU baseline on {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(threads)
+        out["cpu_baseline"] = cpu_baseline(threads, base)
     print(json.dumps(out), flush=True)
 
 
