@@ -167,6 +167,38 @@ float twmc_progress(const void* ctx);         /* fraction of the batch's steps e
 int twmc_noisy_matrix(const twmc_matrix* m, uint64_t n, uint64_t seed, uint64_t traj,
                       uint32_t tag, double* out);
 
+/* ---- batched parameter sweeps -------------------------------------------
+ * Many independent systems (sweep points) of uneven 1D/2D sizes, each with
+ * its own uniform parameters and trajectories, advanced together by one
+ * persistent launch per frame (no reference counterpart: the reference runs
+ * one system per process).  Every trajectory equals the same trajectory of a
+ * twmc_create/twmc_run context with the same parameters and seed. */
+typedef struct twmc_sweep_point {
+  int32_t nx, ny;          /* lattice; N x 1 is stored as 1 x N */
+  uint64_t n_traj;         /* trajectories of this point */
+  uint64_t traj_id0;       /* id of its first trajectory (Philox counter) */
+  double J, gamma;
+  double omega[2], U[2], F[2], beta_init[2], beta_sigma[2];  /* uniform, complex */
+} twmc_sweep_point;
+
+int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt, uint64_t frame_steps,
+                      uint64_t seed, int device, void** ctx_out);
+void twmc_sweep_destroy(void* ctx);
+const char* twmc_sweep_last_error(const void* ctx);
+/* trajectories stacked per work item (1D points; 2D points always 1) */
+int twmc_sweep_set_batching(void* ctx, const int32_t* tb);
+int twmc_sweep_get_batching(void* ctx, int32_t* tb);
+/* times the stacking candidates of every 1D shape for `steps` steps and keeps
+ * the fastest; tb_out (n_points, may be NULL) receives the choice */
+int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out);
+/* vacuum-start run from t_start to t_end with the stored initial frame;
+ * traj_m2 / traj_m4 (total trajectories x n_frames, points in order) receive
+ * the per-trajectory site means of |beta|^2 and |beta|^4; use_graph captures
+ * the frame loop into one hipGraph; kernel_ms receives the HIP-event time */
+int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, double* traj_m2, double* traj_m4,
+                   uint64_t* n_frames_out, double* kernel_ms);
+int twmc_sweep_query(void* ctx, twmc_stats* st);
+
 #ifdef __cplusplus
 }
 #endif

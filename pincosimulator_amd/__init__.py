@@ -5,9 +5,9 @@ Hand-written HIP kernels for gfx950 behind the C ABI in include/twmc_hip.h
 Solver / Task / Manager plugin boundary (pincosimulator_amd/host), and this
 Python front end used by the tests and the benchmark.
 """
-from .engine import Engine, Result, System, frame_schedule, noisy_matrix  # noqa: F401
+from .engine import Engine, Result, Sweep, System, frame_schedule, noisy_matrix  # noqa: F401
 from . import observables  # noqa: F401
 from ._lib import TWMCError, LIB_PATH  # noqa: F401
 
-__all__ = ["Engine", "Result", "System", "frame_schedule", "noisy_matrix", "observables",
+__all__ = ["Engine", "Result", "Sweep", "System", "frame_schedule", "noisy_matrix", "observables",
            "TWMCError", "LIB_PATH"]

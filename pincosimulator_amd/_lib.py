@@ -28,6 +28,8 @@ EXPORTS = [
     "twmc_abi_version", "twmc_last_error", "twmc_create", "twmc_destroy", "twmc_batch_begin",
     "twmc_batch_info_get", "twmc_batch_advance", "twmc_batch_sync", "twmc_batch_read", "twmc_run",
     "twmc_set_timing", "twmc_query", "twmc_reset_stats", "twmc_progress", "twmc_noisy_matrix",
+    "twmc_sweep_create", "twmc_sweep_destroy", "twmc_sweep_last_error", "twmc_sweep_set_batching",
+    "twmc_sweep_get_batching", "twmc_sweep_autotune", "twmc_sweep_run", "twmc_sweep_query",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -83,6 +85,14 @@ class Stats(C.Structure):
     ]
 
 
+class SweepPoint(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("n_traj", C.c_uint64), ("traj_id0", C.c_uint64),
+        ("J", C.c_double), ("gamma", C.c_double), ("omega", C.c_double * 2), ("U", C.c_double * 2),
+        ("F", C.c_double * 2), ("beta_init", C.c_double * 2), ("beta_sigma", C.c_double * 2),
+    ]
+
+
 _lib = None
 
 
@@ -114,6 +124,22 @@ def load():
     L.twmc_progress.restype = C.c_float
     L.twmc_noisy_matrix.argtypes = [C.POINTER(Matrix), C.c_uint64, C.c_uint64, C.c_uint64,
                                     C.c_uint32, _dp]
+    i32p = C.POINTER(C.c_int32)
+    L.twmc_sweep_create.argtypes = [C.POINTER(SweepPoint), C.c_uint64, C.c_double, C.c_uint64, C.c_uint64,
+                                    C.c_int, C.POINTER(vp)]
+    L.twmc_sweep_destroy.argtypes = [vp]
+    L.twmc_sweep_destroy.restype = None
+    L.twmc_sweep_last_error.argtypes = [vp]
+    L.twmc_sweep_last_error.restype = C.c_char_p
+    L.twmc_sweep_set_batching.argtypes = [vp, i32p]
+    L.twmc_sweep_get_batching.argtypes = [vp, i32p]
+    L.twmc_sweep_autotune.argtypes = [vp, C.c_int32, i32p]
+    L.twmc_sweep_run.argtypes = [vp, C.c_double, C.c_double, C.c_int32, _dp, _dp, C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_double)]
+    L.twmc_sweep_query.argtypes = [vp, C.POINTER(Stats)]
+    for name in ("twmc_sweep_create", "twmc_sweep_set_batching", "twmc_sweep_get_batching",
+                 "twmc_sweep_autotune", "twmc_sweep_run", "twmc_sweep_query"):
+        getattr(L, name).restype = C.c_int
     for name in ("twmc_create", "twmc_batch_begin", "twmc_batch_info_get", "twmc_batch_advance",
                  "twmc_batch_sync", "twmc_batch_read", "twmc_run", "twmc_set_timing", "twmc_query",
                  "twmc_reset_stats", "twmc_noisy_matrix"):
@@ -122,7 +148,8 @@ def load():
     return L
 
 
-def check(rc, ctx=None):
+def check(rc, ctx=None, sweep=False):
     if rc != TWMC_OK:
-        msg = load().twmc_last_error(ctx)
+        L = load()
+        msg = L.twmc_sweep_last_error(ctx) if sweep else L.twmc_last_error(ctx)
         raise TWMCError(rc, msg.decode() if msg else "")

@@ -1,0 +1,690 @@
+// twmc_sweep.hip -- a parameter sweep of many uneven lattices (1D chains and
+// 2D squares of different sizes, each point with its own F, detuning, ...)
+// advanced by ONE persistent launch per frame.
+//
+// Work items are (point, block of trajectories).  Workgroups take items from a
+// global counter in decreasing order of cost (largest lattices first), so the
+// uneven items balance across the chip.  An item of a 2D point holds one
+// trajectory; an item of a 1D chain holds tb trajectories stacked as the rows
+// of a tb x N array, transformed along the rows only, so short chains still
+// fill a workgroup.  tb per lattice shape is chosen by twmc_sweep_autotune,
+// which times the candidates on the device.  The per-step arithmetic is the
+// LDS kernel's (same Stockham passes, K product, Philox increments, update),
+// so every trajectory of a sweep equals the same trajectory run alone.
+// The frame loop (counter reset + step kernel per frame) can be captured once
+// into a hipGraph and launched as one graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/twmc_hip.h"
+#include "twmc_cplx.h"
+#include "twmc_internal.h"
+#include "twmc_rng.h"
+#include "twmc_stockham.h"
+
+namespace twmc {
+
+namespace {
+
+constexpr int kSweepBlock = 256;
+constexpr int kSweepMaxSites = 4096;  // sites per work item (LDS and registers)
+constexpr int kPPT = kSweepMaxSites / 2 / kSweepBlock;
+
+struct SweepPointDev {
+  int nx, ny, tb, fft_x;  // fft_x = 1: 2D point (rows are lattice rows), 0: stacked 1D trajectories
+  RadixPlan px, py;
+  int tw_x, tw_y, kc_x, kc_y;  // offsets into the table pools
+  float rsl_re, rsl_im, u_re, u_im, f_re, f_im;
+  float dt4, kscale;
+  int n_traj;
+  uint64_t traj_id0;
+  uint64_t state_off;  // complex elements
+  uint64_t m_off;      // offset of [n_traj][n_frames] per-trajectory means
+};
+
+struct SweepItemDev {
+  int point, t0;
+};
+
+struct SweepArgs {
+  const SweepPointDev* points;
+  const SweepItemDev* items;
+  int n_items;
+  unsigned int* counter;
+  const cpx<float>* tw_pool;
+  const float* kc_pool;
+  cpx<float>* beta;
+  double* traj_m2;
+  double* traj_m4;
+  float dt;
+  uint32_t key0, key1;
+  uint64_t step0;
+  int n_steps, capture, frame, n_frames;
+};
+
+__global__ void sweep_reset_kernel(unsigned int* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+}
+
+template <bool CAPTURE_ONLY>
+__global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* red = reinterpret_cast<double*>(smem);  // [8]
+  cpx<float>* bufA = reinterpret_cast<cpx<float>*>(smem + 64);
+  cpx<float>* bufB = bufA + kSweepMaxSites;
+  cpx<float>* twx = bufB + kSweepMaxSites;    // [512]
+  cpx<float>* twy = twx + 512;                // [512]
+  float* kcx = reinterpret_cast<float*>(twy + 512);  // [512]
+  float* kcy = kcx + 512;                     // [512]
+  __shared__ int s_item;
+
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(g.counter, 1u);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= g.n_items) break;  // every workgroup reaches this once the queue is drained
+    const SweepItemDev item = g.items[it];
+    const SweepPointDev P = g.points[item.point];
+    const int rows_traj = P.fft_x ? 1 : min(P.tb, P.n_traj - item.t0);
+    const int nx = P.fft_x ? P.nx : rows_traj;  // rows of the item array
+    const int ny = P.ny;
+    const int m = nx * ny;
+    const int npairs = m / 2;  // ny is even for every supported size
+    for (int i = threadIdx.x; i < ny; i += blockDim.x) {
+      twy[i] = g.tw_pool[P.tw_y + i];
+      kcy[i] = g.kc_pool[P.kc_y + i];
+    }
+    if (P.fft_x)
+      for (int i = threadIdx.x; i < P.nx; i += blockDim.x) {
+        twx[i] = g.tw_pool[P.tw_x + i];
+        kcx[i] = g.kc_pool[P.kc_x + i];
+      }
+    __syncthreads();
+
+    // trajectory and in-trajectory pair index of each owned pair
+    cpx<float>* gbeta = g.beta + P.state_off + (size_t)item.t0 * (P.fft_x ? m : ny);
+    cpx<float> beta[2 * kPPT];
+#pragma unroll
+    for (int j = 0; j < kPPT; ++j) {
+      const int p = threadIdx.x + kSweepBlock * j;
+      if (p < npairs) {
+        beta[2 * j] = gbeta[2 * p];
+        beta[2 * j + 1] = gbeta[2 * p + 1];
+      }
+    }
+
+    const int iters = CAPTURE_ONLY ? 1 : g.n_steps;
+    for (int s = 0; s < iters; ++s) {
+#pragma unroll
+      for (int j = 0; j < kPPT; ++j) {
+        const int p = threadIdx.x + kSweepBlock * j;
+        if (p < npairs) {
+          bufA[2 * p] = beta[2 * j];
+          bufA[2 * p + 1] = beta[2 * j + 1];
+        }
+      }
+      __syncthreads();
+
+      if (s == 0 && g.capture) {
+        // per-trajectory site means of |beta|^2 and |beta|^4 (FP64)
+        if (P.fft_x) {
+          double a2 = 0.0, a4 = 0.0;
+#pragma unroll
+          for (int j = 0; j < kPPT; ++j) {
+            const int p = threadIdx.x + kSweepBlock * j;
+            if (p < npairs)
+#pragma unroll
+              for (int h = 0; h < 2; ++h) {
+                const cpx<float> z = beta[2 * j + h];
+                const double nn = double(z.re) * double(z.re) + double(z.im) * double(z.im);
+                a2 += nn;
+                a4 += nn * nn;
+              }
+          }
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) {
+            a2 += __shfl_xor(a2, o, 64);
+            a4 += __shfl_xor(a4, o, 64);
+          }
+          if ((threadIdx.x & 63) == 0) {
+            red[2 * (threadIdx.x >> 6)] = a2;
+            red[2 * (threadIdx.x >> 6) + 1] = a4;
+          }
+          __syncthreads();
+          if (threadIdx.x == 0) {
+            double s2 = 0, s4 = 0;
+            for (int w = 0; w < kSweepBlock / 64; ++w) {
+              s2 += red[2 * w];
+              s4 += red[2 * w + 1];
+            }
+            const size_t o = P.m_off + (size_t)item.t0 * g.n_frames + g.frame;
+            g.traj_m2[o] = s2 / double(m);
+            g.traj_m4[o] = s4 / double(m);
+          }
+        } else {
+          for (int r = threadIdx.x; r < rows_traj; r += blockDim.x) {
+            double s2 = 0, s4 = 0;
+            for (int c = 0; c < ny; ++c) {
+              const cpx<float> z = bufA[r * ny + c];
+              const double nn = double(z.re) * double(z.re) + double(z.im) * double(z.im);
+              s2 += nn;
+              s4 += nn * nn;
+            }
+            const size_t o = P.m_off + (size_t)(item.t0 + r) * g.n_frames + g.frame;
+            g.traj_m2[o] = s2 / double(ny);
+            g.traj_m4[o] = s4 / double(ny);
+          }
+        }
+        __syncthreads();
+      }
+      if (CAPTURE_ONLY) break;
+
+      // rows (length ny), then columns (length nx) for 2D points
+      cpx<float>* cur = fft_dim<-1>(bufA, bufB, P.py, nx, ny, 1, twy);
+      if (P.fft_x) {
+        cpx<float>* other = (cur == bufA) ? bufB : bufA;
+        cur = fft_dim<-1>(cur, other, P.px, ny, 1, ny, twx);
+      }
+      for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        const int kx = i / ny, ky = i - kx * ny;
+        const float kim = P.kscale * ((P.fft_x ? kcx[kx] : 0.0f) + kcy[ky]);
+        const cpx<float> f = cur[i];
+        cur[i] = mk(-kim * f.im, kim * f.re);
+      }
+      __syncthreads();
+      {
+        cpx<float>* other = (cur == bufA) ? bufB : bufA;
+        cur = fft_dim<+1>(cur, other, P.py, nx, ny, 1, twy);
+        if (P.fft_x) {
+          other = (cur == bufA) ? bufB : bufA;
+          cur = fft_dim<+1>(cur, other, P.px, ny, 1, ny, twx);
+        }
+      }
+
+      const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
+      const float inv_n = 1.0f / float(P.fft_x ? m : ny);
+      const float dt = g.dt;
+#pragma unroll
+      for (int j = 0; j < kPPT; ++j) {
+        const int p = threadIdx.x + kSweepBlock * j;
+        if (p < npairs) {
+          const int row = P.fft_x ? 0 : (2 * p) / ny;
+          const uint32_t pair = P.fft_x ? (uint32_t)p : (uint32_t)(p - row * (ny / 2));
+          const uint64_t traj = P.traj_id0 + (uint64_t)(item.t0 + row);
+          const u32x4 w = philox4x32_10(make_counter(traj, TAG_WIENER, step, pair), g.key0, g.key1);
+          float xr[2], xi[2];
+          box_muller_f32(w.x, w.y, xr[0], xi[0]);
+          box_muller_f32(w.z, w.w, xr[1], xi[1]);
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int i = 2 * p + h;
+            cpx<float> z = beta[2 * j + h];
+            cpx<float> hh = cur[i];
+            hh = mk(hh.re * inv_n, hh.im * inv_n);
+            const float q = z.re * z.re + z.im * z.im - 1.0f;
+            const cpx<float> c = mk(P.rsl_re + (-P.u_im) * q, P.rsl_im + P.u_re * q);
+            const cpx<float> cb = cmul(c, z);
+            const cpx<float> sdr = mk(cb.re + hh.re + (-P.f_im), cb.im + hh.im + P.f_re);
+            z = mk(z.re + sdr.re * dt + P.dt4 * xr[h], z.im + sdr.im * dt + P.dt4 * xi[h]);
+            beta[2 * j + h] = z;
+          }
+        }
+      }
+      __syncthreads();
+    }
+
+    if (!CAPTURE_ONLY) {
+#pragma unroll
+      for (int j = 0; j < kPPT; ++j) {
+        const int p = threadIdx.x + kSweepBlock * j;
+        if (p < npairs) {
+          gbeta[2 * p] = beta[2 * j];
+          gbeta[2 * p + 1] = beta[2 * j + 1];
+        }
+      }
+    }
+  }
+}
+
+// vacuum-type start beta_0 = beta_init + sigma xi (TWMCBaseSolver.cpp:126-134)
+__global__ void sweep_init_kernel(const SweepPointDev* points, int n_points, const double* binit, cpx<float>* beta,
+                                  uint32_t key0, uint32_t key1) {
+  for (int p = 0; p < n_points; ++p) {
+    const SweepPointDev P = points[p];
+    const int n = P.nx * P.ny;
+    const size_t total = (size_t)P.n_traj * n;
+    const double br = binit[4 * p], bi = binit[4 * p + 1], sr = binit[4 * p + 2], si = binit[4 * p + 3];
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+      const int t = (int)(idx / n), i = (int)(idx - (size_t)t * n);
+      const u32x4 w = philox4x32_10(make_counter(P.traj_id0 + t, TAG_INIT, 0u, (uint32_t)(i >> 1)), key0, key1);
+      double zr, zi;
+      if (i & 1) box_muller_f64(w.z, w.w, zr, zi); else box_muller_f64(w.x, w.y, zr, zi);
+      beta[P.state_off + idx] = mk(float(br + (sr * zr - si * zi)), float(bi + (sr * zi + si * zr)));
+    }
+  }
+}
+
+size_t sweep_lds_bytes() { return 64 + 2 * kSweepMaxSites * 8 + 2 * 512 * 8 + 2 * 512 * 4; }
+
+}  // namespace
+}  // namespace twmc
+
+using namespace twmc;
+
+namespace {
+
+thread_local std::string g_sweep_error;
+
+struct SweepCtx {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double dt = 0.0;
+  uint64_t frame_steps = 1, seed = 0;
+  std::vector<twmc_sweep_point> pts;
+  std::vector<SweepPointDev> dev_pts;
+  std::vector<int> tb;  // trajectories per work item, per point
+  std::vector<cpx<float>> tw_pool;
+  std::vector<float> kc_pool;
+  uint64_t total_traj = 0, total_sites = 0;
+  void *d_points = nullptr, *d_items = nullptr, *d_counter = nullptr, *d_tw = nullptr, *d_kc = nullptr,
+       *d_beta = nullptr, *d_m2 = nullptr, *d_m4 = nullptr, *d_binit = nullptr;
+  size_t m_cap = 0;
+  int n_items = 0, grid = 0;
+  twmc_stats stats{};
+};
+
+int sfail(SweepCtx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  g_sweep_error = m;
+  return code;
+}
+
+#define SCHK(c, expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return sfail(c, TWMC_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+bool plan(int len, RadixPlan& P) {
+  P.len = len;
+  P.npass = 0;
+  int m = len;
+  for (int r : {8, 4, 2, 3, 5})
+    while (m % r == 0) {
+      if (P.npass >= kMaxRadixPasses) return false;
+      P.radix[P.npass++] = r;
+      m /= r;
+    }
+  return m == 1;
+}
+
+int add_tables(SweepCtx* c, int L, int& tw_off, int& kc_off) {
+  tw_off = (int)c->tw_pool.size();
+  kc_off = (int)c->kc_pool.size();
+  for (int m = 0; m < L; ++m) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * m / L;
+    c->tw_pool.push_back(mk(float(cosl(a)), float(sinl(a))));
+    c->kc_pool.push_back(L == 1 ? 0.0f : float(cos(2.0 * M_PI / double(L) * double(m))));
+  }
+  return 0;
+}
+
+// lays out points, work items (largest first) and device buffers for the current tb
+int build_items(SweepCtx* c) {
+  std::vector<std::pair<double, SweepItemDev>> items;
+  for (size_t p = 0; p < c->pts.size(); ++p) {
+    const SweepPointDev& P = c->dev_pts[p];
+    const int step = P.fft_x ? 1 : c->tb[p];
+    for (int t0 = 0; t0 < P.n_traj; t0 += step) {
+      int rows = P.fft_x ? 1 : std::min(step, P.n_traj - t0);
+      double cost = double(rows) * P.nx * P.ny * std::log2(double(P.nx * P.ny) + 1.0);
+      items.push_back({cost, SweepItemDev{(int)p, t0}});
+    }
+  }
+  std::stable_sort(items.begin(), items.end(), [](auto& a, auto& b) { return a.first > b.first; });
+  std::vector<SweepItemDev> flat;
+  for (auto& it : items) flat.push_back(it.second);
+  for (size_t p = 0; p < c->pts.size(); ++p) c->dev_pts[p].tb = c->tb[p];
+  c->n_items = (int)flat.size();
+  if (c->d_items) (void)hipFree(c->d_items);
+  c->d_items = nullptr;
+  SCHK(c, hipMalloc(&c->d_items, flat.size() * sizeof(SweepItemDev)));
+  SCHK(c, hipMemcpy(c->d_items, flat.data(), flat.size() * sizeof(SweepItemDev), hipMemcpyHostToDevice));
+  SCHK(c, hipMemcpy(c->d_points, c->dev_pts.data(), c->dev_pts.size() * sizeof(SweepPointDev),
+                    hipMemcpyHostToDevice));
+  return TWMC_OK;
+}
+
+// one segment: reset the queue, then the persistent step kernel
+hipError_t launch_segment(SweepCtx* c, const SweepArgs& a) {
+  hipLaunchKernelGGL(sweep_reset_kernel, dim3(1), dim3(64), 0, c->stream, (unsigned int*)c->d_counter);
+  if (a.n_steps > 0)
+    hipLaunchKernelGGL(sweep_step_kernel<false>, dim3(c->grid), dim3(kSweepBlock), sweep_lds_bytes(), c->stream, a);
+  else
+    hipLaunchKernelGGL(sweep_step_kernel<true>, dim3(c->grid), dim3(kSweepBlock), sweep_lds_bytes(), c->stream, a);
+  return hipGetLastError();
+}
+
+SweepArgs base_args(SweepCtx* c) {
+  SweepArgs a{};
+  a.points = (const SweepPointDev*)c->d_points;
+  a.items = (const SweepItemDev*)c->d_items;
+  a.n_items = c->n_items;
+  a.counter = (unsigned int*)c->d_counter;
+  a.tw_pool = (const cpx<float>*)c->d_tw;
+  a.kc_pool = (const float*)c->d_kc;
+  a.beta = (cpx<float>*)c->d_beta;
+  a.traj_m2 = (double*)c->d_m2;
+  a.traj_m4 = (double*)c->d_m4;
+  a.dt = (float)c->dt;
+  a.key0 = uint32_t(c->seed & 0xffffffffu);
+  a.key1 = uint32_t(c->seed >> 32);
+  return a;
+}
+
+int init_state(SweepCtx* c) {
+  hipLaunchKernelGGL(sweep_init_kernel, dim3(1024), dim3(256), 0, c->stream, (const SweepPointDev*)c->d_points,
+                     (int)c->pts.size(), (const double*)c->d_binit, (cpx<float>*)c->d_beta,
+                     uint32_t(c->seed & 0xffffffffu), uint32_t(c->seed >> 32));
+  SCHK(c, hipGetLastError());
+  return TWMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* twmc_sweep_last_error(const void* ctx) {
+  return ctx ? static_cast<const SweepCtx*>(ctx)->err.c_str() : g_sweep_error.c_str();
+}
+
+int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt, uint64_t frame_steps,
+                      uint64_t seed, int device, void** ctx_out) {
+  if (!pts || !ctx_out || n_points == 0) return sfail(nullptr, TWMC_ERR_INVALID, "empty sweep");
+  if (!(dt > 0.0) || frame_steps < 1) return sfail(nullptr, TWMC_ERR_INVALID, "dt and frame_steps must be positive");
+  *ctx_out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return sfail(nullptr, TWMC_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= ndev) return sfail(nullptr, TWMC_ERR_INVALID, "device index out of range");
+  SweepCtx* c = new SweepCtx();
+  c->device = device;
+  c->dt = dt;
+  c->frame_steps = frame_steps;
+  c->seed = seed;
+  c->pts.assign(pts, pts + n_points);
+  std::map<std::pair<int, int>, std::pair<int, int>> tables;  // length -> (tw, kc) offsets
+  auto table_for = [&](int L) {
+    auto it = tables.find({L, 0});
+    if (it != tables.end()) return it->second;
+    int a, b;
+    add_tables(c, L, a, b);
+    tables[{L, 0}] = {a, b};
+    return std::make_pair(a, b);
+  };
+  uint64_t state_off = 0, m_off = 0;
+  for (uint64_t p = 0; p < n_points; ++p) {
+    const twmc_sweep_point& q = pts[p];
+    int nx = q.nx, ny = q.ny;
+    if (nx > 1 && ny == 1) std::swap(nx, ny);  // chains are 1 x N
+    SweepPointDev P{};
+    P.nx = nx;
+    P.ny = ny;
+    P.fft_x = nx > 1 ? 1 : 0;
+    if (nx * ny > kSweepMaxSites || ny > 512 || nx > 512 || (ny % 2) != 0 || q.n_traj == 0 ||
+        q.n_traj > (uint64_t)INT32_MAX || !plan(ny, P.py) || !plan(nx, P.px)) {
+      delete c;
+      return sfail(nullptr, TWMC_ERR_UNSUPPORTED,
+                   "sweep point " + std::to_string(p) + ": lattices up to 4096 sites, even row length, sizes of 2,3,5");
+    }
+    auto ty = table_for(ny);
+    auto tx = table_for(nx);
+    P.tw_y = ty.first;
+    P.kc_y = ty.second;
+    P.tw_x = tx.first;
+    P.kc_x = tx.second;
+    // rsl = -i omega - gamma/2 (TWMCBaseSolver.cpp:74)
+    P.rsl_re = float(q.omega[1] - q.gamma / 2.0);
+    P.rsl_im = float(-q.omega[0]);
+    P.u_re = float(q.U[0]);
+    P.u_im = float(q.U[1]);
+    P.f_re = float(q.F[0]);
+    P.f_im = float(q.F[1]);
+    P.dt4 = float(std::sqrt(q.gamma * dt / 4.0));
+    P.kscale = float(-2.0 * q.J);
+    P.n_traj = (int)q.n_traj;
+    P.traj_id0 = q.traj_id0;
+    P.state_off = state_off;
+    P.m_off = m_off;
+    state_off += (uint64_t)q.n_traj * nx * ny;
+    m_off += q.n_traj;  // times n_frames at run time
+    c->dev_pts.push_back(P);
+    // default stacking: fill a workgroup with about 2048 sites
+    c->tb.push_back(P.fft_x ? 1 : std::max(1, std::min((int)q.n_traj, 2048 / ny)));
+  }
+  c->total_traj = m_off;
+  c->total_sites = state_off;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c->d_points, c->dev_pts.size() * sizeof(SweepPointDev));
+  if (e == hipSuccess) e = hipMalloc(&c->d_counter, 256);
+  if (e == hipSuccess) e = hipMalloc(&c->d_tw, c->tw_pool.size() * sizeof(cpx<float>));
+  if (e == hipSuccess) e = hipMalloc(&c->d_kc, c->kc_pool.size() * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&c->d_beta, state_off * sizeof(cpx<float>));
+  std::vector<double> binit(4 * n_points);
+  for (uint64_t p = 0; p < n_points; ++p) {
+    binit[4 * p] = pts[p].beta_init[0];
+    binit[4 * p + 1] = pts[p].beta_init[1];
+    binit[4 * p + 2] = pts[p].beta_sigma[0];
+    binit[4 * p + 3] = pts[p].beta_sigma[1];
+  }
+  if (e == hipSuccess) e = hipMalloc(&c->d_binit, binit.size() * 8);
+  if (e == hipSuccess) e = hipMemcpy(c->d_tw, c->tw_pool.data(), c->tw_pool.size() * sizeof(cpx<float>), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_kc, c->kc_pool.data(), c->kc_pool.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_binit, binit.data(), binit.size() * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipFuncSetAttribute((const void*)sweep_step_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sweep_lds_bytes());
+  if (e == hipSuccess) e = hipFuncSetAttribute((const void*)sweep_step_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sweep_lds_bytes());
+  int per_cu = 0, cus = 0;
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sweep_step_kernel<false>, kSweepBlock, sweep_lds_bytes());
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess) {
+    std::string m = std::string("sweep setup: ") + hipGetErrorString(e);
+    twmc_sweep_destroy(c);
+    return sfail(nullptr, TWMC_ERR_HIP, m);
+  }
+  c->grid = std::max(1, per_cu) * cus;
+  int rc = build_items(c);
+  if (rc) {
+    std::string m = c->err;
+    twmc_sweep_destroy(c);
+    return sfail(nullptr, rc, m);
+  }
+  *ctx_out = c;
+  return TWMC_OK;
+}
+
+void twmc_sweep_destroy(void* ctx) {
+  if (!ctx) return;
+  SweepCtx* c = static_cast<SweepCtx*>(ctx);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void* bufs[] = {c->d_points, c->d_items, c->d_counter, c->d_tw, c->d_kc, c->d_beta, c->d_m2, c->d_m4, c->d_binit};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int twmc_sweep_set_batching(void* ctx, const int32_t* tb) {
+  if (!ctx || !tb) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
+  SweepCtx* c = static_cast<SweepCtx*>(ctx);
+  for (size_t p = 0; p < c->pts.size(); ++p) {
+    const SweepPointDev& P = c->dev_pts[p];
+    int v = P.fft_x ? 1 : tb[p];
+    if (v < 1 || (int64_t)v * P.ny > kSweepMaxSites) return sfail(c, TWMC_ERR_INVALID, "stacking exceeds 4096 sites");
+    c->tb[p] = std::min(v, P.n_traj);
+  }
+  return build_items(c);
+}
+
+int twmc_sweep_get_batching(void* ctx, int32_t* tb) {
+  if (!ctx || !tb) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
+  SweepCtx* c = static_cast<SweepCtx*>(ctx);
+  for (size_t p = 0; p < c->tb.size(); ++p) tb[p] = c->tb[p];
+  return TWMC_OK;
+}
+
+// Runs the sweep from the vacuum start; frames follow the reference loop with
+// store_initial.  use_graph captures the frame loop into one hipGraph.
+int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, double* traj_m2, double* traj_m4,
+                   uint64_t* n_frames_out, double* kernel_ms) {
+  if (!ctx) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
+  SweepCtx* c = static_cast<SweepCtx*>(ctx);
+  SCHK(c, hipSetDevice(c->device));
+  // frame schedule (TWMCBaseSolver.cpp:221-266), store_initial = true
+  double dt_obs = c->dt * (double)c->frame_steps;
+  uint64_t nf = (uint64_t)(size_t)nearbyint((t_end - t_start) / dt_obs) + 1;
+  std::vector<uint64_t> cap{0};
+  uint64_t i_frame = 1, i_step = 0;
+  double t = t_start;
+  while (t <= t_end) {
+    t += c->dt;
+    i_step++;
+    if ((i_step % c->frame_steps == 0) && i_frame < nf) {
+      cap.push_back(i_step);
+      i_frame++;
+      if (i_frame == nf) break;
+    }
+  }
+  const uint64_t frames = i_frame;
+  size_t need = (size_t)c->total_traj * frames;
+  if (need > c->m_cap) {
+    if (c->d_m2) (void)hipFree(c->d_m2);
+    if (c->d_m4) (void)hipFree(c->d_m4);
+    c->d_m2 = c->d_m4 = nullptr;
+    SCHK(c, hipMalloc(&c->d_m2, need * 8));
+    SCHK(c, hipMalloc(&c->d_m4, need * 8));
+    c->m_cap = need;
+  }
+  // per-trajectory means are laid out [point block][traj][frame]
+  std::vector<SweepPointDev> pts = c->dev_pts;
+  for (auto& P : pts) P.m_off *= frames;
+  SCHK(c, hipMemcpyAsync(c->d_points, pts.data(), pts.size() * sizeof(SweepPointDev), hipMemcpyHostToDevice, c->stream));
+  int rc = init_state(c);
+  if (rc) return rc;
+  SCHK(c, hipStreamSynchronize(c->stream));
+
+  hipEvent_t e0, e1;
+  SCHK(c, hipEventCreate(&e0));
+  SCHK(c, hipEventCreate(&e1));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) SCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  else SCHK(c, hipEventRecord(e0, c->stream));
+  for (size_t k = 0; k < cap.size(); ++k) {
+    uint64_t end = (k + 1 < cap.size()) ? cap[k + 1] : i_step;
+    SweepArgs a = base_args(c);
+    a.step0 = cap[k];
+    a.n_steps = (int)(end - cap[k]);
+    a.capture = 1;
+    a.frame = (int)k;
+    a.n_frames = (int)frames;
+    hipError_t le = launch_segment(c, a);
+    if (le != hipSuccess) {
+      if (use_graph) {
+        hipGraph_t g2;
+        (void)hipStreamEndCapture(c->stream, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+      }
+      return sfail(c, TWMC_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(le));
+    }
+  }
+  if (use_graph) {
+    SCHK(c, hipStreamEndCapture(c->stream, &graph));
+    SCHK(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    SCHK(c, hipEventRecord(e0, c->stream));
+    SCHK(c, hipGraphLaunch(exec, c->stream));
+  }
+  SCHK(c, hipEventRecord(e1, c->stream));
+  SCHK(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  SCHK(c, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  c->stats.step_kernel_ms += ms;
+  c->stats.step_launches += cap.size();
+  c->stats.site_steps += 0;
+  for (size_t p = 0; p < c->pts.size(); ++p)
+    c->stats.site_steps += (uint64_t)c->dev_pts[p].n_traj * c->dev_pts[p].nx * c->dev_pts[p].ny * i_step;
+  c->stats.grid = c->grid;
+  c->stats.block = kSweepBlock;
+  c->stats.lds_bytes = (int)sweep_lds_bytes();
+  if (kernel_ms) *kernel_ms = ms;
+  if (n_frames_out) *n_frames_out = frames;
+  if (traj_m2) SCHK(c, hipMemcpy(traj_m2, c->d_m2, need * 8, hipMemcpyDeviceToHost));
+  if (traj_m4) SCHK(c, hipMemcpy(traj_m4, c->d_m4, need * 8, hipMemcpyDeviceToHost));
+  return TWMC_OK;
+}
+
+// Times each candidate stacking per 1D shape on a short run (steps steps of
+// every point of that shape) and keeps the fastest per site-step.
+int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out) {
+  if (!ctx) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
+  SweepCtx* c = static_cast<SweepCtx*>(ctx);
+  if (steps < 1) steps = 20;
+  std::map<int, std::vector<size_t>> by_len;  // 1D chain length -> points
+  for (size_t p = 0; p < c->pts.size(); ++p)
+    if (!c->dev_pts[p].fft_x) by_len[c->dev_pts[p].ny].push_back(p);
+  std::vector<int32_t> best = c->tb;
+  for (auto& kv : by_len) {
+    const int ny = kv.first;
+    double best_rate = -1.0;
+    for (int cand = 1; cand * ny <= kSweepMaxSites; cand *= 2) {
+      if (cand > c->dev_pts[kv.second[0]].n_traj && cand > 1) break;
+      std::vector<int32_t> tb = c->tb;
+      for (size_t p : kv.second) tb[p] = cand;
+      // time only this shape: a context holding just these points
+      std::vector<twmc_sweep_point> sub;
+      for (size_t p : kv.second) sub.push_back(c->pts[p]);
+      void* tctx = nullptr;
+      int rc = twmc_sweep_create(sub.data(), sub.size(), c->dt, (uint64_t)steps, c->seed, c->device, &tctx);
+      if (rc) return sfail(c, rc, twmc_sweep_last_error(nullptr));
+      std::vector<int32_t> subtb(sub.size(), cand);
+      rc = twmc_sweep_set_batching(tctx, subtb.data());
+      double ms = 0.0;
+      if (!rc) rc = twmc_sweep_run(tctx, 0.0, c->dt * steps, 0, nullptr, nullptr, nullptr, &ms);
+      uint64_t ss = static_cast<SweepCtx*>(tctx)->stats.site_steps;
+      twmc_sweep_destroy(tctx);
+      if (rc) return sfail(c, rc, "autotune run failed");
+      double rate = ms > 0 ? double(ss) / ms : 0.0;
+      if (rate > best_rate) {
+        best_rate = rate;
+        for (size_t p : kv.second) best[p] = cand;
+      }
+    }
+  }
+  int rc = twmc_sweep_set_batching(ctx, best.data());
+  if (rc) return rc;
+  if (tb_out)
+    for (size_t p = 0; p < best.size(); ++p) tb_out[p] = best[p];
+  return TWMC_OK;
+}
+
+int twmc_sweep_query(void* ctx, twmc_stats* st) {
+  if (!ctx || !st) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
+  *st = static_cast<SweepCtx*>(ctx)->stats;
+  return TWMC_OK;
+}
+
+}  // extern "C"

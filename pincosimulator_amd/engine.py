@@ -287,3 +287,80 @@ def noisy_matrix(spec, n, seed, traj, tag):
     out = np.zeros(n, dtype=np.complex128)
     check(L.twmc_noisy_matrix(C.byref(m), n, seed, traj, tag, out.ctypes.data_as(C.POINTER(C.c_double))))
     return out
+
+
+class Sweep:
+    """Many uneven systems in one persistent launch per frame (libtwmc_hip.so
+    twmc_sweep_*).  points: list of dicts with nx, ny, n_traj, traj_id0, J,
+    gamma, omega, U, F, beta_init, beta_sigma (uniform, complex allowed)."""
+
+    def __init__(self, points, dt, frame_steps, seed, device=0):
+        L = _lib.load()
+        arr = (_lib.SweepPoint * len(points))()
+        for k, p in enumerate(points):
+            q = arr[k]
+            q.nx, q.ny = int(p["nx"]), int(p["ny"])
+            q.n_traj, q.traj_id0 = int(p["n_traj"]), int(p.get("traj_id0", 0))
+            q.J, q.gamma = float(p.get("J", 0.0)), float(p["gamma"])
+            for name, default in (("omega", 0.0), ("U", 0.0), ("F", 0.0), ("beta_init", 0.0), ("beta_sigma", 0.5)):
+                v = complex(p.get(name, default))
+                getattr(q, name)[0], getattr(q, name)[1] = v.real, v.imag
+        self.points = points
+        self.n_traj = [int(p["n_traj"]) for p in points]
+        ctx = C.c_void_p()
+        check(L.twmc_sweep_create(arr, len(points), dt, frame_steps, seed, device, C.byref(ctx)), None, sweep=True)
+        self._ctx, self._L = ctx, L
+        self.dt, self.frame_steps = dt, frame_steps
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._L.twmc_sweep_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def batching(self):
+        tb = (C.c_int32 * len(self.points))()
+        check(self._L.twmc_sweep_get_batching(self._ctx, tb), self._ctx, sweep=True)
+        return list(tb)
+
+    def set_batching(self, tb):
+        arr = (C.c_int32 * len(self.points))(*[int(v) for v in tb])
+        check(self._L.twmc_sweep_set_batching(self._ctx, arr), self._ctx, sweep=True)
+
+    def autotune(self, steps=20):
+        tb = (C.c_int32 * len(self.points))()
+        check(self._L.twmc_sweep_autotune(self._ctx, steps, tb), self._ctx, sweep=True)
+        return list(tb)
+
+    def run(self, t_end, t_start=0.0, graph=True):
+        """Returns dict(n_frames, kernel_ms, m2, m4) with m2/m4 lists per point of
+        arrays (n_traj, n_frames)."""
+        dt_obs = self.dt * self.frame_steps
+        nf_max = int(round((t_end - t_start) / dt_obs)) + 2
+        tot = sum(self.n_traj)
+        m2 = np.zeros(tot * nf_max)
+        m4 = np.zeros(tot * nf_max)
+        nf = C.c_uint64(0)
+        ms = C.c_double(0.0)
+        check(self._L.twmc_sweep_run(self._ctx, t_start, t_end, int(bool(graph)),
+                                     m2.ctypes.data_as(C.POINTER(C.c_double)),
+                                     m4.ctypes.data_as(C.POINTER(C.c_double)), C.byref(nf), C.byref(ms)),
+              self._ctx, sweep=True)
+        f = int(nf.value)
+        m2, m4 = m2[:tot * f].reshape(tot, f), m4[:tot * f].reshape(tot, f)
+        out2, out4, o = [], [], 0
+        for n in self.n_traj:
+            out2.append(m2[o:o + n])
+            out4.append(m4[o:o + n])
+            o += n
+        return {"n_frames": f, "kernel_ms": ms.value, "m2": out2, "m4": out4}
+
+    def stats(self):
+        st = _lib.Stats()
+        check(self._L.twmc_sweep_query(self._ctx, C.byref(st)), self._ctx, sweep=True)
+        return {f: getattr(st, f) for f, _ in st._fields_}

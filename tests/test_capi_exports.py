@@ -1,0 +1,40 @@
+"""The C-ABI libraries load and export every symbol include/*.h declares (CPU:
+no compute call is made)."""
+import ctypes
+import os
+import re
+import subprocess
+
+from pincosimulator_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"^\s*(?:[A-Za-z_][\w\s\*]*?\b)?(twmc_\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
+
+
+def test_header_declares_what_the_bindings_list():
+    names = declared("twmc_hip.h")
+    assert set(names) == set(_lib.EXPORTS), set(names) ^ set(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared("twmc_hip.h"):
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    for name in declared("twmc_hip.h"):
+        assert re.search(r"\bT " + name + r"\b", out), name
+
+
+def test_abi_version_and_error_paths_without_gpu():
+    L = _lib.load()
+    assert L.twmc_abi_version() == 1
+    # invalid arguments are rejected before any device work
+    assert L.twmc_create(None, 0, None) == -1
+    assert L.twmc_last_error(None)
+    assert L.twmc_batch_advance(None, 1) == -1
