@@ -87,13 +87,62 @@ def cpu_baseline(threads, spec=None, target_s=12.0):
     }
 
 
+SWEEP_SHAPES = [(1, 32), (1, 64), (1, 128), (1, 256), (1, 512), (16, 16), (32, 32), (48, 48), (64, 64)]
+
+
+def sweep_points():
+    """512 points on a 32 x 16 grid of F in [0.5, 4] and detuning in [-2, 3],
+    shapes assigned round-robin, 128 trajectories each, U=0.05, gamma=1, J=1."""
+    import numpy as np
+    Fs = np.linspace(0.5, 4.0, 32)
+    Ds = np.linspace(-2.0, 3.0, 16)
+    pts = []
+    for k in range(512):
+        nx, ny = SWEEP_SHAPES[k % len(SWEEP_SHAPES)]
+        pts.append(dict(nx=nx, ny=ny, n_traj=128, traj_id0=128 * k, J=1.0, gamma=1.0,
+                        omega=float(Ds[k // 32]), U=0.05, F=float(Fs[k % 32]), beta_init=0.0, beta_sigma=0.5))
+    return pts
+
+
+def bench_sweep(args):
+    """One step = one 500-step frame of all 512 points (the full run is 60)."""
+    from pincosimulator_amd import Sweep
+    pts = sweep_points()
+    sw = Sweep(pts, 0.002, 500, 20260103)
+    tb = sw.autotune(steps=50) if args.autotune else sw.batching()
+    log(f"bench: sweep stacking per shape: " + ", ".join(f"{pts[k]['nx']}x{pts[k]['ny']}:{tb[k]}" for k in range(9)))
+    if args.warmup:
+        sw.run(args.warmup * 500 * 0.002, graph=True)
+    t0 = time.perf_counter()
+    out = sw.run(args.steps * 500 * 0.002, graph=True)
+    t1 = time.perf_counter()
+    site_steps = sum(p["n_traj"] * p["nx"] * p["ny"] for p in pts) * 500.0 * args.steps
+    wall = t1 - t0
+    value = site_steps / wall
+    log(f"bench: sweep {args.steps} frames: wall {wall:.3f} s, graph kernel time {out['kernel_ms']:.1f} ms, "
+        f"{value:.4e} site-steps/s")
+    res = {
+        "metric": METRIC, "value": value, "unit": "trajectory·site-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "512-point sweep, F in [0.5,4] x detuning in [-2,3], shapes round-robin "
+                               "{1D 32..512, 2D 16..64}, 128 trajectories each, U=0.05 gamma=1 dt=0.002, "
+                               "one persistent launch per 500-step frame, hipGraph frame loop",
+                   "stacking": {f"{pts[k]['nx']}x{pts[k]['ny']}": tb[k] for k in range(9)},
+                   "kernel_ms": out["kernel_ms"]},
+    }
+    sw.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="square64", choices=["square64", "lieb384"])
+    ap.add_argument("--workload", default="square64", choices=["square64", "lieb384", "sweep512"])
+    ap.add_argument("--autotune", action="store_true", help="sweep512: autotune the stacking first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 256 MiB device-to-device copy (torch) whose HBM counters "
                          "calibrate FETCH_SIZE / WRITE_SIZE in a --pmc profile")
@@ -118,6 +167,8 @@ def main():
         torch.cuda.synchronize()
         log("bench: calibration copy of 256 MiB done")
         del a, b
+    if args.workload == "sweep512":
+        return bench_sweep(args)
     if args.workload == "lieb384":
         base, n_traj, fsteps = lieb_spec(), LIEB_TRAJ, LIEB_FRAME
     else:

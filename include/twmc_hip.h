@@ -193,10 +193,16 @@ int twmc_sweep_get_batching(void* ctx, int32_t* tb);
 int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out);
 /* vacuum-start run from t_start to t_end with the stored initial frame;
  * traj_m2 / traj_m4 (total trajectories x n_frames, points in order) receive
- * the per-trajectory site means of |beta|^2 and |beta|^4; use_graph captures
- * the frame loop into one hipGraph; kernel_ms receives the HIP-event time */
+ * the per-trajectory site means of |beta|^2 and |beta|^4; ksum (n_points x
+ * n_frames x 34) the FP64 sums over trajectories of |DFT(beta)_k|^2 per radial
+ * bin (32 bins, see twmc_sweep_mode_bins) and of sum_i beta_i (Re, Im);
+ * use_graph captures the frame loop into one hipGraph; kernel_ms receives the
+ * HIP-event time.  Any output pointer may be NULL. */
 int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, double* traj_m2, double* traj_m4,
-                   uint64_t* n_frames_out, double* kernel_ms);
+                   double* ksum, uint64_t* n_frames_out, double* kernel_ms);
+/* radial bin (0..31) of every mode of point `point` (nx * ny entries, row-major):
+ * bin 0 is k = 0, others 1 + floor(|k| / (pi sqrt 2 / 31)) capped at 31 */
+int twmc_sweep_mode_bins(void* ctx, uint64_t point, uint8_t* bins);
 int twmc_sweep_query(void* ctx, twmc_stats* st);
 
 #ifdef __cplusplus

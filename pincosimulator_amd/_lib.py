@@ -30,6 +30,7 @@ EXPORTS = [
     "twmc_set_timing", "twmc_query", "twmc_reset_stats", "twmc_progress", "twmc_noisy_matrix",
     "twmc_sweep_create", "twmc_sweep_destroy", "twmc_sweep_last_error", "twmc_sweep_set_batching",
     "twmc_sweep_get_batching", "twmc_sweep_autotune", "twmc_sweep_run", "twmc_sweep_query",
+    "twmc_sweep_mode_bins",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -134,11 +135,12 @@ def load():
     L.twmc_sweep_set_batching.argtypes = [vp, i32p]
     L.twmc_sweep_get_batching.argtypes = [vp, i32p]
     L.twmc_sweep_autotune.argtypes = [vp, C.c_int32, i32p]
-    L.twmc_sweep_run.argtypes = [vp, C.c_double, C.c_double, C.c_int32, _dp, _dp, C.POINTER(C.c_uint64),
+    L.twmc_sweep_run.argtypes = [vp, C.c_double, C.c_double, C.c_int32, _dp, _dp, _dp, C.POINTER(C.c_uint64),
                                  C.POINTER(C.c_double)]
+    L.twmc_sweep_mode_bins.argtypes = [vp, C.c_uint64, C.POINTER(C.c_uint8)]
     L.twmc_sweep_query.argtypes = [vp, C.POINTER(Stats)]
     for name in ("twmc_sweep_create", "twmc_sweep_set_batching", "twmc_sweep_get_batching",
-                 "twmc_sweep_autotune", "twmc_sweep_run", "twmc_sweep_query"):
+                 "twmc_sweep_autotune", "twmc_sweep_run", "twmc_sweep_query", "twmc_sweep_mode_bins"):
         getattr(L, name).restype = C.c_int
     for name in ("twmc_create", "twmc_batch_begin", "twmc_batch_info_get", "twmc_batch_advance",
                  "twmc_batch_sync", "twmc_batch_read", "twmc_run", "twmc_set_timing", "twmc_query",

@@ -36,6 +36,10 @@ namespace {
 constexpr int kSweepBlock = 256;
 constexpr int kSweepMaxSites = 4096;  // sites per work item (LDS and registers)
 constexpr int kPPT = kSweepMaxSites / 2 / kSweepBlock;
+constexpr int kBins = 32;  // radial |k| bins: 0 is k = 0, 1..31 split (0, pi sqrt 2]
+constexpr int kMaxRows = 256;  // stacked trajectories per work item
+constexpr int kMaxNx = 64;     // rows of a 2D point
+constexpr int kMaxNy = 512;    // row length (chain length or 2D row)
 
 struct SweepPointDev {
   int nx, ny, tb, fft_x;  // fft_x = 1: 2D point (rows are lattice rows), 0: stacked 1D trajectories
@@ -47,6 +51,8 @@ struct SweepPointDev {
   uint64_t traj_id0;
   uint64_t state_off;  // complex elements
   uint64_t m_off;      // offset of [n_traj][n_frames] per-trajectory means
+  int bin_off;         // offset of the mode -> radial bin table (one entry per mode of one trajectory)
+  int index;           // point index (offset of its [n_frames][kBins + 2] k-space / mean-field sums)
 };
 
 struct SweepItemDev {
@@ -63,6 +69,8 @@ struct SweepArgs {
   cpx<float>* beta;
   double* traj_m2;
   double* traj_m4;
+  const unsigned char* bin_pool;  // mode -> bin
+  double* ksum;                   // [points][n_frames][kBins + 2]: sum of |DFT beta|^2 per bin, sum of beta
   float dt;
   uint32_t key0, key1;
   uint64_t step0;
@@ -79,11 +87,12 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
   double* red = reinterpret_cast<double*>(smem);  // [8]
   cpx<float>* bufA = reinterpret_cast<cpx<float>*>(smem + 64);
   cpx<float>* bufB = bufA + kSweepMaxSites;
-  cpx<float>* twx = bufB + kSweepMaxSites;    // [512]
-  cpx<float>* twy = twx + 512;                // [512]
-  float* kcx = reinterpret_cast<float*>(twy + 512);  // [512]
-  float* kcy = kcx + 512;                     // [512]
+  cpx<float>* twx = bufB + kSweepMaxSites;    // [kMaxNx]
+  cpx<float>* twy = twx + kMaxNx;             // [kMaxNy]
+  float* kcx = reinterpret_cast<float*>(twy + kMaxNy);  // [kMaxNx]
+  float* kcy = kcx + kMaxNx;                  // [kMaxNy]
   __shared__ int s_item;
+  __shared__ double s_bins[kBins + 2];
 
   for (;;) {
     __syncthreads();
@@ -133,8 +142,12 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
       }
       __syncthreads();
 
-      if (s == 0 && g.capture) {
-        // per-trajectory site means of |beta|^2 and |beta|^4 (FP64)
+      const bool cap = (s == 0 && g.capture);
+      if (cap) {
+        // per-trajectory site means of |beta|^2 and |beta|^4 (FP64), summed
+        // in a fixed order so that repeated runs agree bit for bit; for
+        // stacked chains every row of the item is a trajectory
+        if (threadIdx.x < kBins + 2) s_bins[threadIdx.x] = 0.0;
         if (P.fft_x) {
           double a2 = 0.0, a4 = 0.0;
 #pragma unroll
@@ -183,9 +196,29 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
             g.traj_m4[o] = s4 / double(ny);
           }
         }
-        __syncthreads();
+        // ensemble sum of beta over all sites (mean field)
+        double br = 0.0, bi = 0.0;
+#pragma unroll
+        for (int j = 0; j < kPPT; ++j) {
+          const int p = threadIdx.x + kSweepBlock * j;
+          if (p < npairs)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              br += double(beta[2 * j + h].re);
+              bi += double(beta[2 * j + h].im);
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          br += __shfl_xor(br, o, 64);
+          bi += __shfl_xor(bi, o, 64);
+        }
+        __syncthreads();  // s_bins cleared, bufA reads done before the FFT overwrites it
+        if ((threadIdx.x & 63) == 0) {
+          atomicAdd(&s_bins[kBins], br);
+          atomicAdd(&s_bins[kBins + 1], bi);
+        }
       }
-      if (CAPTURE_ONLY) break;
 
       // rows (length ny), then columns (length nx) for 2D points
       cpx<float>* cur = fft_dim<-1>(bufA, bufB, P.py, nx, ny, 1, twy);
@@ -193,6 +226,21 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
         cpx<float>* other = (cur == bufA) ? bufB : bufA;
         cur = fft_dim<-1>(cur, other, P.px, ny, 1, ny, twx);
       }
+
+      if (cap) {
+        // |DFT beta|^2 per radial bin, summed over this item's trajectories
+        for (int i = threadIdx.x; i < m; i += blockDim.x) {
+          const cpx<float> f = cur[i];
+          const int mode = P.fft_x ? i : i % ny;
+          atomicAdd(&s_bins[g.bin_pool[P.bin_off + mode]], double(f.re) * double(f.re) + double(f.im) * double(f.im));
+        }
+        __syncthreads();
+        if (threadIdx.x < kBins + 2)
+          atomicAdd(&g.ksum[((size_t)P.index * g.n_frames + g.frame) * (kBins + 2) + threadIdx.x], s_bins[threadIdx.x]);
+        __syncthreads();
+      }
+      if (CAPTURE_ONLY) break;
+
       for (int i = threadIdx.x; i < m; i += blockDim.x) {
         const int kx = i / ny, ky = i - kx * ny;
         const float kim = P.kscale * ((P.fft_x ? kcx[kx] : 0.0f) + kcy[ky]);
@@ -273,7 +321,7 @@ __global__ void sweep_init_kernel(const SweepPointDev* points, int n_points, con
   }
 }
 
-size_t sweep_lds_bytes() { return 64 + 2 * kSweepMaxSites * 8 + 2 * 512 * 8 + 2 * 512 * 4; }
+size_t sweep_lds_bytes() { return 64 + 2 * kSweepMaxSites * 8 + (kMaxNx + kMaxNy) * (8 + 4); }
 
 }  // namespace
 }  // namespace twmc
@@ -295,9 +343,12 @@ struct SweepCtx {
   std::vector<int> tb;  // trajectories per work item, per point
   std::vector<cpx<float>> tw_pool;
   std::vector<float> kc_pool;
+  std::vector<unsigned char> bin_pool;
   uint64_t total_traj = 0, total_sites = 0;
   void *d_points = nullptr, *d_items = nullptr, *d_counter = nullptr, *d_tw = nullptr, *d_kc = nullptr,
-       *d_beta = nullptr, *d_m2 = nullptr, *d_m4 = nullptr, *d_binit = nullptr;
+       *d_beta = nullptr, *d_m2 = nullptr, *d_m4 = nullptr, *d_binit = nullptr, *d_bins = nullptr,
+       *d_ksum = nullptr;
+  size_t ksum_cap = 0;
   size_t m_cap = 0;
   int n_items = 0, grid = 0;
   twmc_stats stats{};
@@ -337,6 +388,26 @@ int add_tables(SweepCtx* c, int L, int& tw_off, int& kc_off) {
     c->kc_pool.push_back(L == 1 ? 0.0f : float(cos(2.0 * M_PI / double(L) * double(m))));
   }
   return 0;
+}
+
+// radial bin of every mode of an nx x ny lattice: bin 0 is k = 0; the other
+// modes go to 1 + floor(|k| / w) (at most 31) with w = pi sqrt 2 / 31 and
+// |k| = 2 pi sqrt((kx'/nx)^2 + (ky'/ny)^2), kx' = min(kx, nx - kx)
+int add_bins(SweepCtx* c, int nx, int ny) {
+  int off = (int)c->bin_pool.size();
+  const double w = M_PI * std::sqrt(2.0) / 31.0;
+  for (int kx = 0; kx < nx; ++kx)
+    for (int ky = 0; ky < ny; ++ky) {
+      int ax = std::min(kx, nx - kx), ay = std::min(ky, ny - ky);
+      if (ax == 0 && ay == 0) {
+        c->bin_pool.push_back(0);
+        continue;
+      }
+      double k = 2.0 * M_PI * std::sqrt(double(ax) * ax / (double(nx) * nx) + double(ay) * ay / (double(ny) * ny));
+      int b = 1 + std::min(30, (int)std::floor(k / w));
+      c->bin_pool.push_back((unsigned char)b);
+    }
+  return off;
 }
 
 // lays out points, work items (largest first) and device buffers for the current tb
@@ -386,6 +457,8 @@ SweepArgs base_args(SweepCtx* c) {
   a.beta = (cpx<float>*)c->d_beta;
   a.traj_m2 = (double*)c->d_m2;
   a.traj_m4 = (double*)c->d_m4;
+  a.bin_pool = (const unsigned char*)c->d_bins;
+  a.ksum = (double*)c->d_ksum;
   a.dt = (float)c->dt;
   a.key0 = uint32_t(c->seed & 0xffffffffu);
   a.key1 = uint32_t(c->seed >> 32);
@@ -423,6 +496,7 @@ int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt,
   c->seed = seed;
   c->pts.assign(pts, pts + n_points);
   std::map<std::pair<int, int>, std::pair<int, int>> tables;  // length -> (tw, kc) offsets
+  std::map<std::pair<int, int>, int> bin_tables;                // shape -> bin table offset
   auto table_for = [&](int L) {
     auto it = tables.find({L, 0});
     if (it != tables.end()) return it->second;
@@ -440,11 +514,12 @@ int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt,
     P.nx = nx;
     P.ny = ny;
     P.fft_x = nx > 1 ? 1 : 0;
-    if (nx * ny > kSweepMaxSites || ny > 512 || nx > 512 || (ny % 2) != 0 || q.n_traj == 0 ||
+    if (nx * ny > kSweepMaxSites || ny > kMaxNy || nx > kMaxNx || (ny % 2) != 0 || q.n_traj == 0 ||
         q.n_traj > (uint64_t)INT32_MAX || !plan(ny, P.py) || !plan(nx, P.px)) {
       delete c;
       return sfail(nullptr, TWMC_ERR_UNSUPPORTED,
-                   "sweep point " + std::to_string(p) + ": lattices up to 4096 sites, even row length, sizes of 2,3,5");
+                   "sweep point " + std::to_string(p) + ": up to 4096 sites, 64 rows, rows of even length <= 512, "
+                   "sizes with factors 2, 3, 5");
     }
     auto ty = table_for(ny);
     auto tx = table_for(nx);
@@ -465,11 +540,17 @@ int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt,
     P.traj_id0 = q.traj_id0;
     P.state_off = state_off;
     P.m_off = m_off;
+    P.index = (int)p;
+    {
+      auto bt = bin_tables.find({nx, ny});
+      if (bt == bin_tables.end()) bt = bin_tables.emplace(std::make_pair(nx, ny), add_bins(c, nx, ny)).first;
+      P.bin_off = bt->second;
+    }
     state_off += (uint64_t)q.n_traj * nx * ny;
     m_off += q.n_traj;  // times n_frames at run time
     c->dev_pts.push_back(P);
     // default stacking: fill a workgroup with about 2048 sites
-    c->tb.push_back(P.fft_x ? 1 : std::max(1, std::min((int)q.n_traj, 2048 / ny)));
+    c->tb.push_back(P.fft_x ? 1 : std::max(1, std::min({(int)q.n_traj, 2048 / ny, kMaxRows})));
   }
   c->total_traj = m_off;
   c->total_sites = state_off;
@@ -488,6 +569,8 @@ int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt,
     binit[4 * p + 3] = pts[p].beta_sigma[1];
   }
   if (e == hipSuccess) e = hipMalloc(&c->d_binit, binit.size() * 8);
+  if (e == hipSuccess) e = hipMalloc(&c->d_bins, c->bin_pool.size());
+  if (e == hipSuccess) e = hipMemcpy(c->d_bins, c->bin_pool.data(), c->bin_pool.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_tw, c->tw_pool.data(), c->tw_pool.size() * sizeof(cpx<float>), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_kc, c->kc_pool.data(), c->kc_pool.size() * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_binit, binit.data(), binit.size() * 8, hipMemcpyHostToDevice);
@@ -517,7 +600,8 @@ void twmc_sweep_destroy(void* ctx) {
   SweepCtx* c = static_cast<SweepCtx*>(ctx);
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* bufs[] = {c->d_points, c->d_items, c->d_counter, c->d_tw, c->d_kc, c->d_beta, c->d_m2, c->d_m4, c->d_binit};
+  void* bufs[] = {c->d_points, c->d_items, c->d_counter, c->d_tw, c->d_kc, c->d_beta, c->d_m2, c->d_m4, c->d_binit,
+                  c->d_bins, c->d_ksum};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -530,7 +614,8 @@ int twmc_sweep_set_batching(void* ctx, const int32_t* tb) {
   for (size_t p = 0; p < c->pts.size(); ++p) {
     const SweepPointDev& P = c->dev_pts[p];
     int v = P.fft_x ? 1 : tb[p];
-    if (v < 1 || (int64_t)v * P.ny > kSweepMaxSites) return sfail(c, TWMC_ERR_INVALID, "stacking exceeds 4096 sites");
+    if (v < 1 || v > kMaxRows || (int64_t)v * P.ny > kSweepMaxSites)
+      return sfail(c, TWMC_ERR_INVALID, "stacking exceeds 4096 sites or 256 trajectories");
     c->tb[p] = std::min(v, P.n_traj);
   }
   return build_items(c);
@@ -546,7 +631,7 @@ int twmc_sweep_get_batching(void* ctx, int32_t* tb) {
 // Runs the sweep from the vacuum start; frames follow the reference loop with
 // store_initial.  use_graph captures the frame loop into one hipGraph.
 int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, double* traj_m2, double* traj_m4,
-                   uint64_t* n_frames_out, double* kernel_ms) {
+                   double* ksum, uint64_t* n_frames_out, double* kernel_ms) {
   if (!ctx) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
   SweepCtx* c = static_cast<SweepCtx*>(ctx);
   SCHK(c, hipSetDevice(c->device));
@@ -575,6 +660,14 @@ int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, d
     SCHK(c, hipMalloc(&c->d_m4, need * 8));
     c->m_cap = need;
   }
+  const size_t kneed = c->pts.size() * frames * (kBins + 2);
+  if (kneed > c->ksum_cap) {
+    if (c->d_ksum) (void)hipFree(c->d_ksum);
+    c->d_ksum = nullptr;
+    SCHK(c, hipMalloc(&c->d_ksum, kneed * 8));
+    c->ksum_cap = kneed;
+  }
+  SCHK(c, hipMemsetAsync(c->d_ksum, 0, kneed * 8, c->stream));  // before any graph capture
   // per-trajectory means are laid out [point block][traj][frame]
   std::vector<SweepPointDev> pts = c->dev_pts;
   for (auto& P : pts) P.m_off *= frames;
@@ -634,6 +727,16 @@ int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, d
   if (n_frames_out) *n_frames_out = frames;
   if (traj_m2) SCHK(c, hipMemcpy(traj_m2, c->d_m2, need * 8, hipMemcpyDeviceToHost));
   if (traj_m4) SCHK(c, hipMemcpy(traj_m4, c->d_m4, need * 8, hipMemcpyDeviceToHost));
+  if (ksum) SCHK(c, hipMemcpy(ksum, c->d_ksum, kneed * 8, hipMemcpyDeviceToHost));
+  return TWMC_OK;
+}
+
+int twmc_sweep_mode_bins(void* ctx, uint64_t point, uint8_t* bins) {
+  if (!ctx || !bins) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
+  SweepCtx* c = static_cast<SweepCtx*>(ctx);
+  if (point >= c->pts.size()) return sfail(c, TWMC_ERR_INVALID, "point index out of range");
+  const SweepPointDev& P = c->dev_pts[point];
+  std::memcpy(bins, &c->bin_pool[P.bin_off], (size_t)P.nx * P.ny);
   return TWMC_OK;
 }
 
@@ -650,7 +753,7 @@ int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out) {
   for (auto& kv : by_len) {
     const int ny = kv.first;
     double best_rate = -1.0;
-    for (int cand = 1; cand * ny <= kSweepMaxSites; cand *= 2) {
+    for (int cand = 1; cand * ny <= kSweepMaxSites && cand <= kMaxRows; cand *= 2) {
       if (cand > c->dev_pts[kv.second[0]].n_traj && cand > 1) break;
       std::vector<int32_t> tb = c->tb;
       for (size_t p : kv.second) tb[p] = cand;
@@ -663,7 +766,7 @@ int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out) {
       std::vector<int32_t> subtb(sub.size(), cand);
       rc = twmc_sweep_set_batching(tctx, subtb.data());
       double ms = 0.0;
-      if (!rc) rc = twmc_sweep_run(tctx, 0.0, c->dt * steps, 0, nullptr, nullptr, nullptr, &ms);
+      if (!rc) rc = twmc_sweep_run(tctx, 0.0, c->dt * steps, 0, nullptr, nullptr, nullptr, nullptr, &ms);
       uint64_t ss = static_cast<SweepCtx*>(tctx)->stats.site_steps;
       twmc_sweep_destroy(tctx);
       if (rc) return sfail(c, rc, "autotune run failed");

@@ -338,18 +338,20 @@ class Sweep:
         return list(tb)
 
     def run(self, t_end, t_start=0.0, graph=True):
-        """Returns dict(n_frames, kernel_ms, m2, m4) with m2/m4 lists per point of
-        arrays (n_traj, n_frames)."""
+        """Returns dict(n_frames, kernel_ms, m2, m4, ksum): m2/m4 are per point
+        arrays (n_traj, n_frames); ksum is (n_points, n_frames, 34) with the
+        32 radial-bin sums of |DFT beta|^2 and the sum of beta (Re, Im)."""
         dt_obs = self.dt * self.frame_steps
         nf_max = int(round((t_end - t_start) / dt_obs)) + 2
         tot = sum(self.n_traj)
         m2 = np.zeros(tot * nf_max)
         m4 = np.zeros(tot * nf_max)
+        ks = np.zeros(len(self.points) * nf_max * 34)
         nf = C.c_uint64(0)
         ms = C.c_double(0.0)
-        check(self._L.twmc_sweep_run(self._ctx, t_start, t_end, int(bool(graph)),
-                                     m2.ctypes.data_as(C.POINTER(C.c_double)),
-                                     m4.ctypes.data_as(C.POINTER(C.c_double)), C.byref(nf), C.byref(ms)),
+        dp = C.POINTER(C.c_double)
+        check(self._L.twmc_sweep_run(self._ctx, t_start, t_end, int(bool(graph)), m2.ctypes.data_as(dp),
+                                     m4.ctypes.data_as(dp), ks.ctypes.data_as(dp), C.byref(nf), C.byref(ms)),
               self._ctx, sweep=True)
         f = int(nf.value)
         m2, m4 = m2[:tot * f].reshape(tot, f), m4[:tot * f].reshape(tot, f)
@@ -358,7 +360,18 @@ class Sweep:
             out2.append(m2[o:o + n])
             out4.append(m4[o:o + n])
             o += n
-        return {"n_frames": f, "kernel_ms": ms.value, "m2": out2, "m4": out4}
+        ks = ks[:len(self.points) * f * 34].reshape(len(self.points), f, 34)
+        return {"n_frames": f, "kernel_ms": ms.value, "m2": out2, "m4": out4, "ksum": ks}
+
+    def mode_bins(self, point):
+        p = self.points[point]
+        nx, ny = int(p["nx"]), int(p["ny"])
+        if nx > 1 and ny == 1:
+            nx, ny = 1, nx
+        out = np.zeros(nx * ny, dtype=np.uint8)
+        check(self._L.twmc_sweep_mode_bins(self._ctx, point, out.ctypes.data_as(C.POINTER(C.c_uint8))),
+              self._ctx, sweep=True)
+        return out.reshape(nx, ny)
 
     def stats(self):
         st = _lib.Stats()

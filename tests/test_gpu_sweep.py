@@ -45,6 +45,28 @@ def test_sweep_points_equal_single_runs():
     sw.close()
 
 
+def test_sweep_kspace_bins_and_mean_field():
+    """Radial-bin sums of |DFT beta|^2 and the mean-field sums equal the same
+    quantities binned from a single run's per-mode ensemble sums."""
+    t_end = 0.2
+    sw = Sweep(points(), DT, FS, SEED)
+    out = sw.run(t_end, graph=False)
+    for k, p in enumerate(points()):
+        bins = sw.mode_bins(k).reshape(-1)
+        assert bins[0] == 0 and np.count_nonzero(bins == 0) == 1 and bins.max() <= 31
+        s = System(nx=p["nx"], ny=p["ny"], dt=DT, gamma=p["gamma"], J=p["J"], omega=p["omega"], U=p["U"],
+                   F=p["F"], beta_init=0.0, beta_sigma=0.5, frame_steps=FS, seed=SEED, t_start=0.0, t_end=t_end)
+        r = Engine(s, precision=32, kernel="generic").run(np.arange(p["n_traj"]) + p["traj_id0"])
+        ref = np.zeros((r.n_frames, 32))
+        for b in range(32):
+            ref[:, b] = r.sum_nk[:, bins == b].sum(axis=1)
+        got = out["ksum"][k, :, :32]
+        assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref)), k
+        mf = r.sum_beta.sum(axis=1)
+        assert np.max(np.abs(out["ksum"][k, :, 32] + 1j * out["ksum"][k, :, 33] - mf)) <= 1e-5 * np.max(np.abs(mf)) + 1e-6
+    sw.close()
+
+
 def test_graph_replays_equal_eager():
     sw = Sweep(points(), DT, FS, SEED)
     eager = sw.run(0.3, graph=False)
