@@ -59,7 +59,13 @@ enum {
 enum { TWMC_NOISE_NONE = 0, TWMC_NOISE_UNIFORM = 1, TWMC_NOISE_GAUSSIAN = 2 };
 enum { TWMC_HOP_FFT = 0, TWMC_HOP_MATRIX = 1 };
 enum { TWMC_ONSITE_OMEGA = 0, TWMC_ONSITE_DETUNING_MINUS_E = 1 };
-enum { TWMC_INIT_SETTINGS = 0, TWMC_INIT_FIXEDPOINT = 1, TWMC_INIT_PREVIOUS = 2 };
+enum {
+  TWMC_INIT_SETTINGS = 0,   /* beta_init + beta_init_sigma . xi (TWMCBaseSolver.cpp:126-134) */
+  TWMC_INIT_FIXEDPOINT = 1, /* beta_init (TWMCBaseSolver.cpp:154-174) */
+  TWMC_INIT_PREVIOUS = 2,   /* caller's state (TWMCBaseSolver.cpp:176-202) */
+  TWMC_INIT_GAUSSIAN = 3    /* Gaussian Wigner sample around a predicted steady state (FFT hopping only):
+                               DFT(beta)_k = N mean [k = 0] + sqrt(N (n_k + 1/2) / 2) (xi_re + i xi_im) */
+};
 enum {
   TWMC_KERNEL_AUTO = 0,     /* fastest kernel that supports the parameters */
   TWMC_KERNEL_GENERIC = 1,  /* LDS-resident mixed-radix kernel, any supported size */
@@ -107,6 +113,9 @@ typedef struct twmc_batch_desc {
   uint64_t step0;             /* absolute index of the first step (resume) */
   double t_start, t_end;
   uint32_t outputs;           /* TWMC_OUT_* mask */
+  const double* gauss_mean;   /* TWMC_INIT_GAUSSIAN: uniform mean field (re, im) */
+  const double* gauss_nk;     /* TWMC_INIT_GAUSSIAN: N populations n_k >= -1/2 of the fluctuations,
+                                 row-major k in DFT order (k = 0 first) */
 } twmc_batch_desc;
 
 typedef struct twmc_batch_info {
@@ -204,6 +213,38 @@ int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, d
  * bin 0 is k = 0, others 1 + floor(|k| / (pi sqrt 2 / 31)) capped at 31 */
 int twmc_sweep_mode_bins(void* ctx, uint64_t point, uint8_t* bins);
 int twmc_sweep_query(void* ctx, twmc_stats* st);
+
+/* ---- learned warm start: the network --------------------------------------
+ * A fully connected MLP (n_in -> n_hidden_layers x n_hidden, exact-erf GELU ->
+ * n_out, linear), FP32, trained and evaluated on the GPU by the library's own
+ * kernels (no reference counterpart).  Parameters are one flat FP32 vector:
+ * for each layer l, W_l [in_l][out_l] row-major, then b_l [out_l].  Rows of x,
+ * t, w, y are samples (row-major).  The loss is sum_ij w_ij (y_ij - t_ij)^2 /
+ * rows (w = NULL: all ones); training is minibatch Adam over a shuffled epoch
+ * (the remainder rows % batch of each epoch is skipped). */
+typedef struct twmc_mlp_config {
+  int32_t n_in, n_hidden, n_hidden_layers, n_out;
+  uint64_t seed;            /* Glorot-uniform initial weights, zero biases */
+} twmc_mlp_config;
+
+typedef struct twmc_mlp_train_opts {
+  int32_t epochs, batch;
+  float lr, beta1, beta2, eps;
+  uint64_t shuffle_seed;
+} twmc_mlp_train_opts;
+
+int twmc_mlp_create(const twmc_mlp_config* cfg, int device, void** ctx_out);
+void twmc_mlp_destroy(void* ctx);
+const char* twmc_mlp_last_error(const void* ctx);
+uint64_t twmc_mlp_n_params(const void* ctx);
+int twmc_mlp_get_params(void* ctx, float* out);
+int twmc_mlp_set_params(void* ctx, const float* in);   /* also clears the Adam moments */
+int twmc_mlp_forward(void* ctx, const float* x, uint64_t rows, float* y);
+int twmc_mlp_loss_grad(void* ctx, const float* x, const float* t, const float* w, uint64_t rows, double* loss,
+                       float* grad);
+/* epoch_loss (epochs entries, may be NULL): mean minibatch loss per epoch */
+int twmc_mlp_train(void* ctx, const float* x, const float* t, const float* w, uint64_t rows,
+                   const twmc_mlp_train_opts* opts, double* epoch_loss);
 
 #ifdef __cplusplus
 }

@@ -26,7 +26,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libtwmc_oracle.so")
 _lib = None
 
-TAG_WIENER, TAG_INIT, TAG_U, TAG_OMEGA, TAG_F, TAG_BINIT, TAG_BSIGMA, TAG_BOND = range(8)
+TAG_WIENER, TAG_INIT, TAG_U, TAG_OMEGA, TAG_F, TAG_BINIT, TAG_BSIGMA, TAG_BOND, TAG_GAUSS = range(9)
 NOISE_NONE, NOISE_UNIFORM, NOISE_GAUSSIAN = 0, 1, 2
 LAT_FFT, LAT_LIEB, LAT_DENSE = 0, 1, 2
 INIT_SETTINGS, INIT_FIXEDPOINT, INIT_PREVIOUS = 0, 1, 2
@@ -93,6 +93,7 @@ def lib():
         L.orc_lieb_coupling.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_int32, dp]
         L.orc_time_batch.argtypes = [C.POINTER(OrcSystem), C.c_uint64, C.c_int32, C.c_uint64, C.c_int32]
         L.orc_time_batch.restype = C.c_double
+        L.orc_gaussian_state.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_uint64, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -134,6 +135,16 @@ def fft(x, sign=-1):
     out = np.empty_like(x)
     rc = lib().orc_fft(nx, ny, _dp(x.view(np.float64)), _dp(out.view(np.float64)), sign)
     assert rc == 0
+    return out
+
+
+def gaussian_state(nx, ny, seed, traj, mean, nk):
+    """Warm-start Gaussian sample (nx, ny) complex for one trajectory id."""
+    m = np.array([complex(mean).real, complex(mean).imag])
+    nk = np.ascontiguousarray(np.asarray(nk, dtype=np.float64).reshape(-1))
+    assert nk.size == nx * ny
+    out = np.empty((nx, ny), dtype=np.complex128)
+    lib().orc_gaussian_state(nx, ny, seed, traj, _dp(m), _dp(nk), _dp(out.view(np.float64)))
     return out
 
 

@@ -432,6 +432,21 @@ void orc_lieb_coupling(int32_t ny, double J_AB, double J_BC, int32_t pbc, double
 }
 
 // TWMCBaseSolver::Compute / TWMCLiebSolver::Compute for one task.
+void orc_gaussian_state(int nx, int ny, uint64_t seed, uint64_t traj, const double mean[2],
+                        const double* nk, double* out) {
+  const size_t n = (size_t)nx * ny;
+  std::vector<cd> k(n), r(n);
+  for (size_t i = 0; i < n; ++i)
+    k[i] = std::sqrt(double(n) * (nk[i] + 0.5) / 2.0) * draw_normal(seed, traj, ORC_TAG_GAUSS, 0, i);
+  k[0] += double(n) * cd(mean[0], mean[1]);
+  fft2(nx, ny, k.data(), r.data(), +1);
+  const double inv_n = 1.0 / double(n);
+  for (size_t i = 0; i < n; ++i) {
+    out[2 * i] = r[i].real() * inv_n;
+    out[2 * i + 1] = r[i].imag() * inv_n;
+  }
+}
+
 int64_t orc_run_trajectory(const orc_system* sp, uint64_t traj, int32_t init_mode,
                            const double* beta_prev, int32_t store_initial, uint64_t step0,
                            double t_start, double t_end, double* traj_out, uint64_t max_frames,

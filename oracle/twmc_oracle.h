@@ -19,6 +19,7 @@
  *                        (TWMCLiebSolver.cpp:29-104, :109-299)
  *   orc_noisy_matrix     NoisyMatrix::Generate / EvaluateNoise (NoisyMatrix.cpp:118-160)
  *   orc_lieb_coupling    Lieb coupling matrix (TWMCLiebSolver.cpp:52-85)
+ *   orc_gaussian_state   warm-start Gaussian sample (no reference counterpart)
  *   orc_time_batch       one trajectory per task over a std::thread pool, as
  *                        ThreadedTaskProcessor/WorkerThread do, in the
  *                        reference's own per-step structure (timing only)
@@ -42,12 +43,22 @@ enum {
   ORC_TAG_F = 4,       /* disorder of F                                          */
   ORC_TAG_BINIT = 5,   /* disorder of beta_init                                  */
   ORC_TAG_BSIGMA = 6,  /* disorder of beta_init_sigma                            */
-  ORC_TAG_BOND = 7     /* bond (hopping) disorder                                */
+  ORC_TAG_BOND = 7,    /* bond (hopping) disorder                                */
+  ORC_TAG_GAUSS = 8    /* Gaussian warm-start sample, counter site = k-mode      */
 };
 
 enum { ORC_NOISE_NONE = 0, ORC_NOISE_UNIFORM = 1, ORC_NOISE_GAUSSIAN = 2 };
 enum { ORC_LAT_FFT = 0, ORC_LAT_LIEB = 1, ORC_LAT_DENSE = 2 };
 enum { ORC_INIT_SETTINGS = 0, ORC_INIT_FIXEDPOINT = 1, ORC_INIT_PREVIOUS = 2 };
+
+/* Gaussian Wigner state around a predicted steady state (no reference
+ * counterpart: the learned warm start's initial sample).  k-space, unnormalised
+ * forward DFT convention:
+ *   DFT(beta)_k = N mean [k = 0] + sqrt(N (nk[k] + 1/2) / 2) (xi_re + i xi_im),
+ *   xi from Philox (tag ORC_TAG_GAUSS, step 0, site = k);  beta = IDFT / N.
+ * Then <|DFT(beta)_k|^2>/N - 1/2 = nk[k] (k != 0) and <beta> = mean. */
+void orc_gaussian_state(int nx, int ny, uint64_t seed, uint64_t traj, const double mean[2],
+                        const double* nk, double* out);
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

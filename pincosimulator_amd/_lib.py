@@ -18,7 +18,7 @@ ERR_NAMES = {-1: "TWMC_ERR_INVALID", -2: "TWMC_ERR_HIP", -3: "TWMC_ERR_NOMEM",
 NOISE_NONE, NOISE_UNIFORM, NOISE_GAUSSIAN = 0, 1, 2
 HOP_FFT, HOP_MATRIX = 0, 1
 ONSITE_OMEGA, ONSITE_DETUNING_MINUS_E = 0, 1
-INIT_SETTINGS, INIT_FIXEDPOINT, INIT_PREVIOUS = 0, 1, 2
+INIT_SETTINGS, INIT_FIXEDPOINT, INIT_PREVIOUS, INIT_GAUSSIAN = 0, 1, 2, 3
 KERNEL_AUTO, KERNEL_GENERIC, KERNEL_FUSED64, KERNEL_MFMA = 0, 1, 2, 3
 KERNEL_NAMES = {0: "auto", 1: "generic", 2: "fused64", 3: "mfma"}
 OUT_OBS, OUT_TRAJ, OUT_FINAL = 1, 2, 4
@@ -31,6 +31,8 @@ EXPORTS = [
     "twmc_sweep_create", "twmc_sweep_destroy", "twmc_sweep_last_error", "twmc_sweep_set_batching",
     "twmc_sweep_get_batching", "twmc_sweep_autotune", "twmc_sweep_run", "twmc_sweep_query",
     "twmc_sweep_mode_bins",
+    "twmc_mlp_create", "twmc_mlp_destroy", "twmc_mlp_last_error", "twmc_mlp_n_params", "twmc_mlp_get_params",
+    "twmc_mlp_set_params", "twmc_mlp_forward", "twmc_mlp_loss_grad", "twmc_mlp_train",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -62,6 +64,7 @@ class BatchDesc(C.Structure):
         ("traj_ids", C.POINTER(C.c_uint64)), ("n_traj", C.c_uint64), ("init_mode", C.c_int32),
         ("store_initial", C.c_int32), ("beta_prev", _dp), ("step0", C.c_uint64),
         ("t_start", C.c_double), ("t_end", C.c_double), ("outputs", C.c_uint32),
+        ("gauss_mean", _dp), ("gauss_nk", _dp),
     ]
 
 
@@ -92,6 +95,16 @@ class SweepPoint(C.Structure):
         ("J", C.c_double), ("gamma", C.c_double), ("omega", C.c_double * 2), ("U", C.c_double * 2),
         ("F", C.c_double * 2), ("beta_init", C.c_double * 2), ("beta_sigma", C.c_double * 2),
     ]
+
+
+class MlpConfig(C.Structure):
+    _fields_ = [("n_in", C.c_int32), ("n_hidden", C.c_int32), ("n_hidden_layers", C.c_int32),
+                ("n_out", C.c_int32), ("seed", C.c_uint64)]
+
+
+class MlpTrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("shuffle_seed", C.c_uint64)]
 
 
 _lib = None
@@ -142,6 +155,22 @@ def load():
     for name in ("twmc_sweep_create", "twmc_sweep_set_batching", "twmc_sweep_get_batching",
                  "twmc_sweep_autotune", "twmc_sweep_run", "twmc_sweep_query", "twmc_sweep_mode_bins"):
         getattr(L, name).restype = C.c_int
+    fp = C.POINTER(C.c_float)
+    L.twmc_mlp_create.argtypes = [C.POINTER(MlpConfig), C.c_int, C.POINTER(vp)]
+    L.twmc_mlp_destroy.argtypes = [vp]
+    L.twmc_mlp_destroy.restype = None
+    L.twmc_mlp_last_error.argtypes = [vp]
+    L.twmc_mlp_last_error.restype = C.c_char_p
+    L.twmc_mlp_n_params.argtypes = [vp]
+    L.twmc_mlp_n_params.restype = C.c_uint64
+    L.twmc_mlp_get_params.argtypes = [vp, fp]
+    L.twmc_mlp_set_params.argtypes = [vp, fp]
+    L.twmc_mlp_forward.argtypes = [vp, fp, C.c_uint64, fp]
+    L.twmc_mlp_loss_grad.argtypes = [vp, fp, fp, fp, C.c_uint64, _dp, fp]
+    L.twmc_mlp_train.argtypes = [vp, fp, fp, fp, C.c_uint64, C.POINTER(MlpTrainOpts), _dp]
+    for name in ("twmc_mlp_create", "twmc_mlp_get_params", "twmc_mlp_set_params", "twmc_mlp_forward",
+                 "twmc_mlp_loss_grad", "twmc_mlp_train"):
+        getattr(L, name).restype = C.c_int
     for name in ("twmc_create", "twmc_batch_begin", "twmc_batch_info_get", "twmc_batch_advance",
                  "twmc_batch_sync", "twmc_batch_read", "twmc_run", "twmc_set_timing", "twmc_query",
                  "twmc_reset_stats", "twmc_noisy_matrix"):
@@ -150,8 +179,11 @@ def load():
     return L
 
 
-def check(rc, ctx=None, sweep=False):
+def check(rc, ctx=None, sweep=False, mlp=False):
     if rc != TWMC_OK:
         L = load()
-        msg = L.twmc_sweep_last_error(ctx) if sweep else L.twmc_last_error(ctx)
+        if mlp:
+            msg = L.twmc_mlp_last_error(ctx)
+        else:
+            msg = L.twmc_sweep_last_error(ctx) if sweep else L.twmc_last_error(ctx)
         raise TWMCError(rc, msg.decode() if msg else "")

@@ -68,7 +68,7 @@ struct Ctx {
   int noise[5] = {0, 0, 0, 0, 0};
   // device-resident constants
   DevBuf d_base[5], d_amp[5], d_twx, d_twy, d_kcx, d_kcy, d_cscp, d_cscr, d_cscv, d_perm_site,
-      d_perm_k, d_ahi, d_alo;
+      d_perm_k, d_ahi, d_alo, d_kamp;
   int np = 0;  // padded site count of the MFMA kernel
   // per-batch buffers
   DevBuf d_ids, d_beta, d_rsl, d_U, d_F, d_partials, d_obs, d_m2, d_m4, d_traj, d_prev;
@@ -331,10 +331,10 @@ int twmc_create(const twmc_params* p, int device, void** ctx_out) {
     if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "upload constant table");
     return ee;
   };
-  up(c->d_twx, twx, true);
-  up(c->d_twy, twy, true);
-  up(c->d_kcx, kcx, false);
-  up(c->d_kcy, kcy, false);
+  (void)up(c->d_twx, twx, true);  // failures are collected in fail_rc
+  (void)up(c->d_twy, twy, true);
+  (void)up(c->d_kcx, kcx, false);
+  (void)up(c->d_kcy, kcy, false);
   for (int k = 0; k < 5 && fail_rc == TWMC_OK; ++k) {
     hipError_t ee = upload(c->d_base[k], c->base[k].data(), c->base[k].size() * 8, s);
     if (ee == hipSuccess) ee = upload(c->d_amp[k], c->amp[k].data(), c->amp[k].size() * 8, s);
@@ -361,7 +361,7 @@ int twmc_create(const twmc_params* p, int device, void** ctx_out) {
     }
     hipError_t ee = upload(c->d_cscp, ptr.data(), ptr.size() * sizeof(int), s);
     if (ee == hipSuccess) ee = upload(c->d_cscr, row.data(), row.size() * sizeof(int), s);
-    if (ee == hipSuccess) up(c->d_cscv, val, false);
+    if (ee == hipSuccess) (void)up(c->d_cscv, val, false);
     if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "upload coupling");
   }
   if (fail_rc == TWMC_OK && c->kernel == TWMC_KERNEL_MFMA) {
@@ -412,7 +412,7 @@ void twmc_destroy(void* ctx) {
     (void)hipEventDestroy(pr.second);
   }
   DevBuf* bufs[] = {&c->d_twx, &c->d_twy, &c->d_kcx, &c->d_kcy, &c->d_cscp, &c->d_cscr,
-                    &c->d_cscv, &c->d_perm_site, &c->d_perm_k, &c->d_ahi, &c->d_alo, &c->d_ids, &c->d_beta, &c->d_rsl,
+                    &c->d_cscv, &c->d_perm_site, &c->d_perm_k, &c->d_ahi, &c->d_alo, &c->d_kamp, &c->d_ids, &c->d_beta, &c->d_rsl,
                     &c->d_U, &c->d_F, &c->d_partials, &c->d_obs, &c->d_m2, &c->d_m4, &c->d_traj,
                     &c->d_prev};
   for (DevBuf* b : bufs) b->release();
@@ -429,7 +429,21 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
   Ctx* c = static_cast<Ctx*>(ctx);
   if (d->n_traj == 0 || !d->traj_ids) return fail(c, TWMC_ERR_INVALID, "empty trajectory batch");
   if (d->n_traj > (uint64_t)INT32_MAX) return fail(c, TWMC_ERR_INVALID, "batch too large");
-  if (d->init_mode < 0 || d->init_mode > 2) return fail(c, TWMC_ERR_INVALID, "unknown init mode");
+  if (d->init_mode < 0 || d->init_mode > 3) return fail(c, TWMC_ERR_INVALID, "unknown init mode");
+  std::vector<double> kamp;
+  if (d->init_mode == TWMC_INIT_GAUSSIAN) {
+    if (c->p.hopping != TWMC_HOP_FFT)
+      return fail(c, TWMC_ERR_UNSUPPORTED, "TWMC_INIT_GAUSSIAN needs FFT (uniform) hopping");
+    if (!d->gauss_mean || !d->gauss_nk) return fail(c, TWMC_ERR_INVALID, "TWMC_INIT_GAUSSIAN needs gauss_mean and gauss_nk");
+    if (!std::isfinite(d->gauss_mean[0]) || !std::isfinite(d->gauss_mean[1]))
+      return fail(c, TWMC_ERR_INVALID, "gauss_mean is not finite");
+    kamp.resize(c->n);
+    for (int k = 0; k < c->n; ++k) {
+      const double v = d->gauss_nk[k] + 0.5;
+      if (!std::isfinite(v) || v < 0.0) return fail(c, TWMC_ERR_INVALID, "gauss_nk must be finite and >= -1/2");
+      kamp[k] = std::sqrt(double(c->n) * v / 2.0);
+    }
+  }
   if (d->init_mode == TWMC_INIT_PREVIOUS && !d->beta_prev)
     return fail(c, TWMC_ERR_INVALID, "TWMC_INIT_PREVIOUS needs beta_prev");
   HIPCHK(c, hipSetDevice(c->device));
@@ -437,6 +451,7 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
   c->desc = *d;
   c->desc.traj_ids = nullptr;
   c->desc.beta_prev = nullptr;
+  c->desc.gauss_mean = c->desc.gauss_nk = nullptr;
   c->n_traj = d->n_traj;
   plan_segments(c);
   c->next_seg = 0;
@@ -484,6 +499,7 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
   if (d->outputs & TWMC_OUT_TRAJ) HIPCHK(c, c->d_traj.ensure(B * nf * n * c->es));
   if (d->init_mode == TWMC_INIT_PREVIOUS)
     HIPCHK(c, upload(c->d_prev, d->beta_prev, B * n * 2 * sizeof(double), s));
+  if (d->init_mode == TWMC_INIT_GAUSSIAN) HIPCHK(c, upload(c->d_kamp, kamp.data(), kamp.size() * 8, s));
 
   InitArgs ia{};
   ia.n = n;
@@ -507,6 +523,27 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
   ia.U = c->d_U.p;
   ia.F = c->d_F.p;
   HIPCHK(c, launch_init(ia, c->precision, s));
+  if (d->init_mode == TWMC_INIT_GAUSSIAN) {
+    if (gauss_lds_bytes(c->p.nx, c->p.ny, c->precision) > 160 * 1024)
+      return fail(c, TWMC_ERR_UNSUPPORTED, "lattice too large for the Gaussian initial state");
+    GaussArgs ga{};
+    ga.n = n;
+    ga.nx = c->p.nx;
+    ga.ny = c->p.ny;
+    ga.n_traj = (int)B;
+    ga.px = c->px;
+    ga.py = c->py;
+    ga.twx = c->d_twx.p;
+    ga.twy = c->d_twy.p;
+    ga.kamp = static_cast<const double*>(c->d_kamp.p);
+    ga.mean_re = d->gauss_mean[0];
+    ga.mean_im = d->gauss_mean[1];
+    ga.key0 = ia.key0;
+    ga.key1 = ia.key1;
+    ga.traj_ids = ia.traj_ids;
+    ga.beta = c->d_beta.p;
+    HIPCHK(c, launch_gauss_init(ga, c->precision, (int)std::min<size_t>(B, (size_t)cus * 4), s));
+  }
   c->batch_active = true;
   return TWMC_OK;
 }

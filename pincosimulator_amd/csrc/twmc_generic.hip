@@ -277,10 +277,10 @@ __global__ void init_kernel(InitArgs a) {
       beta[i] = mk(T(br + pr), T(bi + pi));
     } else if (a.mode == 1) {
       beta[i] = mk(T(a.base[3][2 * i]), T(a.base[3][2 * i + 1]));
-    } else {
+    } else if (a.mode == 2) {
       const double* bp = a.beta_prev + 2 * ((size_t)b * a.n + i);
       beta[i] = mk(T(bp[0]), T(bp[1]));
-    }
+    }  // mode 3: gauss_init_kernel writes the state
     if (a.per_traj_params || b == 0) {
       size_t off = (size_t)(a.per_traj_params ? b : 0) * a.n + i;
       double orr, oi, ur, ui, fr, fi;
@@ -302,6 +302,44 @@ __global__ void init_kernel(InitArgs a) {
       reinterpret_cast<cpx<T>*>(a.U)[off] = mk(T(ur), T(ui));
       reinterpret_cast<cpx<T>*>(a.F)[off] = mk(T(fr), T(fi));
     }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) gauss_init_kernel(GaussArgs g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = g.n;
+  cpx<T>* bufA = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* bufB = bufA + n;
+  cpx<T>* twx = bufB + n;
+  cpx<T>* twy = twx + g.nx;
+  for (int i = threadIdx.x; i < g.nx; i += blockDim.x) twx[i] = reinterpret_cast<const cpx<T>*>(g.twx)[i];
+  for (int i = threadIdx.x; i < g.ny; i += blockDim.x) twy[i] = reinterpret_cast<const cpx<T>*>(g.twy)[i];
+  __syncthreads();
+  const T inv_n = T(1.0 / double(n));
+  for (int b = blockIdx.x; b < g.n_traj; b += gridDim.x) {
+    const uint64_t traj = g.traj_ids[b];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      const u32x4 x = philox4x32_10(make_counter(traj, TAG_GAUSS, 0u, (uint32_t)(k >> 1)), g.key0, g.key1);
+      double zr, zi;
+      if (k & 1) box_muller_f64(x.z, x.w, zr, zi); else box_muller_f64(x.x, x.y, zr, zi);
+      double re = g.kamp[k] * zr, im = g.kamp[k] * zi;
+      if (k == 0) {
+        re += double(n) * g.mean_re;
+        im += double(n) * g.mean_im;
+      }
+      bufA[k] = mk(T(re), T(im));
+    }
+    __syncthreads();
+    // inverse transform: rows (length ny), then columns (length nx)
+    cpx<T>* cur = fft_dim<+1>(bufA, bufB, g.py, g.nx, g.ny, 1, twy);
+    if (g.nx > 1) {
+      cpx<T>* other = (cur == bufA) ? bufB : bufA;
+      cur = fft_dim<+1>(cur, other, g.px, g.ny, 1, g.ny, twx);
+    }
+    cpx<T>* out = reinterpret_cast<cpx<T>*>(g.beta) + (size_t)b * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) out[i] = mk(cur[i].re * inv_n, cur[i].im * inv_n);
+    __syncthreads();  // the next trajectory overwrites bufA
   }
 }
 
@@ -404,6 +442,25 @@ hipError_t launch_init(const InitArgs& a, int precision, hipStream_t s) {
     hipLaunchKernelGGL(init_kernel<double>, dim3((unsigned)grid), dim3(block), 0, s, a);
   else
     hipLaunchKernelGGL(init_kernel<float>, dim3((unsigned)grid), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+
+size_t gauss_lds_bytes(int nx, int ny, int precision) {
+  const size_t es = precision == 64 ? 16 : 8;
+  return (2 * (size_t)nx * ny + nx + ny) * es;
+}
+
+hipError_t launch_gauss_init(const GaussArgs& a, int precision, int grid, hipStream_t s) {
+  const size_t lds = gauss_lds_bytes(a.nx, a.ny, precision);
+  const void* f = precision == 64 ? (const void*)gauss_init_kernel<double> : (const void*)gauss_init_kernel<float>;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  if (precision == 64)
+    hipLaunchKernelGGL(gauss_init_kernel<double>, dim3((unsigned)grid), dim3(256), lds, s, a);
+  else
+    hipLaunchKernelGGL(gauss_init_kernel<float>, dim3((unsigned)grid), dim3(256), lds, s, a);
   return hipGetLastError();
 }
 

@@ -76,6 +76,20 @@ struct InitArgs {
   void* F;
 };
 
+// Gaussian Wigner sample around a predicted steady state: per trajectory,
+// DFT(beta)_k = N mean [k = 0] + kamp[k] (xi_re + i xi_im), beta = IDFT / N.
+struct GaussArgs {
+  int n, nx, ny, n_traj;
+  RadixPlan px, py;
+  const void* twx;           // [nx] complex (T): exp(-2 pi i m / nx)
+  const void* twy;           // [ny] complex (T)
+  const double* kamp;        // [n] standard deviation per quadrature of DFT(beta)_k
+  double mean_re, mean_im;
+  uint32_t key0, key1;
+  const uint64_t* traj_ids;
+  void* beta;                // out [n_traj][n] (T)
+};
+
 struct ReduceArgs {
   int n, slots;
   const double* partials_in;  // [slots][kQ][n]
@@ -86,6 +100,8 @@ struct ReduceArgs {
 };
 
 hipError_t launch_init(const InitArgs& a, int precision, hipStream_t s);
+hipError_t launch_gauss_init(const GaussArgs& a, int precision, int grid, hipStream_t s);
+size_t gauss_lds_bytes(int nx, int ny, int precision);
 hipError_t launch_generic(const StepArgs& a, int precision, int grid, int block, size_t lds,
                           hipStream_t s);
 size_t generic_lds_bytes(int nx, int ny, int precision);

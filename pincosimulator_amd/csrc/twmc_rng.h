@@ -30,7 +30,8 @@ enum : uint32_t {
   TAG_F = 4,
   TAG_BINIT = 5,
   TAG_BSIGMA = 6,
-  TAG_BOND = 7
+  TAG_BOND = 7,
+  TAG_GAUSS = 8  // Gaussian warm-start sample, counter pair = k-mode pair
 };
 
 struct u32x4 {

@@ -20,7 +20,7 @@ from ._lib import TWMCError, check
 _KERNELS = {"auto": _lib.KERNEL_AUTO, "generic": _lib.KERNEL_GENERIC, "fused64": _lib.KERNEL_FUSED64,
             "mfma": _lib.KERNEL_MFMA}
 _INITS = {"settings": _lib.INIT_SETTINGS, "fixedpoint": _lib.INIT_FIXEDPOINT,
-          "previous": _lib.INIT_PREVIOUS}
+          "previous": _lib.INIT_PREVIOUS, "gaussian": _lib.INIT_GAUSSIAN}
 
 
 def frame_schedule(t_start, t_end, timestep, frames_freq=0.0, frames_period=0.0, n_frames=0):
@@ -198,7 +198,9 @@ class Engine:
 
     # --- staged interface (device-resident; used by the benchmark) -------------
     def begin(self, traj_ids, t_start=None, t_end=None, init="settings", beta_prev=None,
-              store_initial=True, step0=0, traj=False, final=False):
+              store_initial=True, step0=0, traj=False, final=False, gauss_mean=None, gauss_nk=None):
+        """init="gaussian" samples DFT(beta)_k = N gauss_mean [k=0] + sqrt(N (n_k + 1/2) / 2) xi
+        with gauss_nk = n_k over the (nx, ny) k-grid in DFT order."""
         s = self.system
         ids = np.ascontiguousarray(np.asarray(traj_ids, dtype=np.uint64).reshape(-1))
         d = _lib.BatchDesc()
@@ -213,6 +215,13 @@ class Engine:
             d.beta_prev = bp.ctypes.data_as(C.POINTER(C.c_double))
         else:
             d.beta_prev = C.POINTER(C.c_double)()
+        if init == "gaussian":
+            self._gm = np.array([complex(gauss_mean).real, complex(gauss_mean).imag])
+            self._gk = np.ascontiguousarray(np.asarray(gauss_nk, dtype=np.float64).reshape(-1))
+            if self._gk.size != self.system.nx * self.system.ny:
+                raise ValueError("gauss_nk needs one population per lattice mode")
+            d.gauss_mean = self._gm.ctypes.data_as(C.POINTER(C.c_double))
+            d.gauss_nk = self._gk.ctypes.data_as(C.POINTER(C.c_double))
         d.step0 = int(step0)
         d.t_start = s.t_start if t_start is None else t_start
         d.t_end = s.t_end if t_end is None else t_end
