@@ -102,6 +102,15 @@ typedef struct twmc_params {
   const double* coupling; /* TWMC_HOP_MATRIX: N x N real, row-major */
   uint64_t seed;          /* Philox key */
   uint64_t frame_steps;   /* steps between stored frames */
+  /* time-dependent drive F(t) (TWMCLiebSolver.cpp:252-256 with
+   * NoisyMatrix::GetAtTimeWithSuggestion, NoisyMatrix.cpp:164-264): when
+   * F_n_times > 0 it replaces F before every step.  F_times ascending,
+   * F_table F_n_times x N complex with gaps already filled; F(t) =
+   * F_table[0] for t <= F_times[0], linear between table times, held at the
+   * last entry after the last time.  Generic and MFMA kernels. */
+  uint64_t F_n_times;
+  const double* F_times;
+  const double* F_table;
 } twmc_params;
 
 typedef struct twmc_batch_desc {

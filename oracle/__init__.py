@@ -52,6 +52,7 @@ class OrcSystem(C.Structure):
         ("coupling", C.POINTER(C.c_double)),
         ("frame_steps", C.c_uint64),
         ("seed", C.c_uint64),
+        ("F_n_times", C.c_uint64), ("F_times", C.POINTER(C.c_double)), ("F_table", C.POINTER(C.c_double)),
     ]
 
 
@@ -218,7 +219,8 @@ def _matrix(keep, spec):
 
 def make_system(p, keep):
     """p: dict with nx, ny, lattice, dt, gamma, J, omega, U, F, beta_init, beta_sigma,
-    frame_steps, seed and optionally pbc, J_AB, J_BC, detuning, E, coupling."""
+    frame_steps, seed and optionally pbc, J_AB, J_BC, detuning, E, coupling,
+    F_t = (times, table n_times x N) for a time-dependent drive."""
     s = OrcSystem()
     s.nx, s.ny = int(p["nx"]), int(p["ny"])
     s.lattice = int(p.get("lattice", LAT_FFT))
@@ -240,6 +242,15 @@ def make_system(p, keep):
         s.coupling = C.POINTER(C.c_double)()
     s.frame_steps = int(p["frame_steps"])
     s.seed = int(p["seed"])
+    if p.get("F_t") is not None:
+        times, table = p["F_t"]
+        times = np.ascontiguousarray(times, dtype=np.float64)
+        table = np.ascontiguousarray(np.asarray(table, dtype=np.complex128).reshape(times.size, -1))
+        assert table.shape[1] == s.nx * s.ny * (3 if s.lattice == LAT_LIEB else 1)
+        keep.bufs += [times, table]
+        s.F_n_times, s.F_times, s.F_table = times.size, _dp(times), _dp(table.view(np.float64))
+    else:
+        s.F_n_times, s.F_times, s.F_table = 0, C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
     return s
 
 
@@ -248,7 +259,7 @@ def run_trajectory(p, traj, t_start=None, t_end=None, init_mode=INIT_SETTINGS, b
     """Returns (frames [n_frames, N] complex128, beta_final [N], t_final)."""
     keep = _Keep()
     s = make_system(p, keep)
-    n = s.nx * s.ny
+    n = s.nx * s.ny * (3 if s.lattice == LAT_LIEB else 1)
     t_start = p.get("t_start", 0.0) if t_start is None else t_start
     t_end = p["t_end"] if t_end is None else t_end
     dt_obs = s.dt * s.frame_steps

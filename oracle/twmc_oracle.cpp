@@ -7,6 +7,7 @@
 
 #include <atomic>
 #include <chrono>
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -249,7 +250,8 @@ struct Trajectory {
   std::vector<cd> f_out, g, h;
 
   Trajectory(const orc_system& s_, uint64_t traj) : s(s_) {
-    n = (size_t)s.nx * s.ny;
+    // a Lieb chain of ny cells has 3 ny sites (TWMCLiebSolver.cpp:29-104)
+    n = (size_t)s.nx * s.ny * (s.lattice == ORC_LAT_LIEB ? 3 : 1);
     fft = (s.lattice == ORC_LAT_FFT);
     if (fft) {
       // TWMCBaseSolver::Setup: K(i,j) = -i 2 J (fx cos(2pi/nx i) + fy cos(2pi/ny j))
@@ -283,6 +285,33 @@ struct Trajectory {
     U = generate(s.U, n, s.seed, traj, ORC_TAG_U);
     F = generate(s.F, n, s.seed, traj, ORC_TAG_F);
     f_out.resize(n); g.resize(n); h.resize(n);
+    // F(t) table and per-interval slopes (NoisyMatrix.cpp:228-239); the slope
+    // after the last time is zero, so the last value is held
+    for (uint64_t k = 0; k < s.F_n_times; ++k) {
+      ft_times.push_back(s.F_times[k]);
+      std::vector<cd> v(n);
+      for (size_t i = 0; i < n; ++i) v[i] = cd(s.F_table[2 * (k * n + i)], s.F_table[2 * (k * n + i) + 1]);
+      ft_vals.push_back(v);
+    }
+    for (size_t k = 0; k < ft_times.size(); ++k) {
+      std::vector<cd> d(n, cd(0.0, 0.0));
+      if (k + 1 < ft_times.size())
+        for (size_t i = 0; i < n; ++i) d[i] = (ft_vals[k + 1][i] - ft_vals[k][i]) / (ft_times[k + 1] - ft_times[k]);
+      ft_slopes.push_back(d);
+    }
+  }
+
+  std::vector<double> ft_times;
+  std::vector<std::vector<cd>> ft_vals, ft_slopes;
+
+  // F at time t (NoisyMatrix::GetAtTimeWithSuggestion, NoisyMatrix.cpp:252-264)
+  void drive_at(double t) {
+    if (t <= ft_times[0]) {
+      F = ft_vals[0];
+      return;
+    }
+    size_t id = size_t(std::lower_bound(ft_times.begin(), ft_times.end(), t) - ft_times.begin()) - 1;
+    for (size_t i = 0; i < n; ++i) F[i] = ft_vals[id][i] + cd(t - ft_times[id], 0.0) * ft_slopes[id][i];
   }
 
   // hopping field h (Base: FFT^-1 (K . FFT beta) / N; Lieb: -i (beta . C))
@@ -485,6 +514,7 @@ int64_t orc_run_trajectory(const orc_system* sp, uint64_t traj, int32_t init_mod
   std::vector<cd> xi(n);
   double t = t_start;
   while (t <= t_end) {
+    if (!tr.ft_times.empty()) tr.drive_at(t);  // TWMCLiebSolver.cpp:252-256
     normal_field(s.seed, traj, ORC_TAG_WIENER, uint32_t(step0 + i_step), n, xi.data());
     tr.step(beta.data(), xi.data());
     t += s.dt;

@@ -111,7 +111,8 @@ typedef struct orc_matrix {
 } orc_matrix;
 
 typedef struct orc_system {
-  int32_t nx, ny;        /* lattice after 1D normalisation (nx = 1 for chains) */
+  int32_t nx, ny;        /* lattice after 1D normalisation (nx = 1 for chains);
+                            ORC_LAT_LIEB: nx = 1, ny cells of 3 sites */
   int32_t lattice;       /* ORC_LAT_* */
   int32_t pbc;           /* Lieb only: periodic closing bond */
   double dt, gamma, J, J_AB, J_BC, detuning;
@@ -120,6 +121,14 @@ typedef struct orc_system {
   const double* coupling; /* ORC_LAT_DENSE: N x N real row-major coupling C */
   uint64_t frame_steps;
   uint64_t seed;         /* Philox key */
+  /* time-dependent drive (TWMCLiebSolver.cpp:252-256 with
+   * NoisyMatrix::GetAtTimeWithSuggestion, NoisyMatrix.cpp:164-264): when
+   * F_n_times > 0, F(t) replaces F before every step; F_table holds
+   * F_n_times x N complex (gaps already filled), F_times ascending; F is
+   * F_table[0] up to F_times[0], linear in between, held after the last time */
+  uint64_t F_n_times;
+  const double* F_times;
+  const double* F_table;
 } orc_system;
 
 /* One trajectory, exactly as the reference's Compute loop runs it, with

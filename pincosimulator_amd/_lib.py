@@ -56,6 +56,7 @@ class Params(C.Structure):
         ("dt", C.c_double), ("gamma", C.c_double), ("J", C.c_double), ("detuning", C.c_double),
         ("omega", Matrix), ("U", Matrix), ("F", Matrix), ("beta_init", Matrix), ("beta_sigma", Matrix),
         ("coupling", _dp), ("seed", C.c_uint64), ("frame_steps", C.c_uint64),
+        ("F_n_times", C.c_uint64), ("F_times", _dp), ("F_table", _dp),
     ]
 
 

@@ -68,7 +68,8 @@ struct Ctx {
   int noise[5] = {0, 0, 0, 0, 0};
   // device-resident constants
   DevBuf d_base[5], d_amp[5], d_twx, d_twy, d_kcx, d_kcy, d_cscp, d_cscr, d_cscv, d_perm_site,
-      d_perm_k, d_ahi, d_alo, d_kamp;
+      d_perm_k, d_ahi, d_alo, d_kamp, d_ft_times, d_ft_vals, d_ft_slopes;
+  int ft_n = 0;  // F(t) table times (0: static F)
   int np = 0;  // padded site count of the MFMA kernel
   // per-batch buffers
   DevBuf d_ids, d_beta, d_rsl, d_U, d_F, d_partials, d_obs, d_m2, d_m4, d_traj, d_prev;
@@ -204,7 +205,7 @@ int choose_kernel(Ctx* c) {
     for (int i = 1; i < c->n && uniform; ++i)
       uniform = c->base[k][2 * i] == c->base[k][0] && c->base[k][2 * i + 1] == c->base[k][1];
   bool fused_ok = fused64_supported(p.nx, p.ny, p.hopping, c->precision,
-                                    (c->noisy_params || !uniform) ? 1 : 0);
+                                    (c->noisy_params || !uniform) ? 1 : 0) && c->ft_n == 0;
   size_t lds = generic_lds_bytes(p.nx, p.ny, c->precision);
   bool generic_ok = generic_pairs_per_thread(c->n, 256) > 0 && lds <= 160 * 1024 &&
                     (p.hopping == TWMC_HOP_MATRIX || (c->px.npass >= 0 && c->py.npass >= 0));
@@ -274,6 +275,24 @@ int twmc_create(const twmc_params* p, int device, void** ctx_out) {
     }
     expand_matrix(*mats[k], c->n, c->base[k], c->amp[k], c->noise[k]);
   }
+  if (p->F_n_times > 0) {
+    if (!p->F_times || !p->F_table) {
+      delete c;
+      return fail(nullptr, TWMC_ERR_INVALID, "F_n_times > 0 needs F_times and F_table");
+    }
+    for (uint64_t k = 0; k < p->F_n_times; ++k)
+      if (!std::isfinite(p->F_times[k]) || (k > 0 && !(p->F_times[k] > p->F_times[k - 1]))) {
+        delete c;
+        return fail(nullptr, TWMC_ERR_INVALID, "F_times must be finite and strictly ascending");
+      }
+    for (uint64_t e = 0; e < 2 * p->F_n_times * (uint64_t)c->n; ++e)
+      if (!std::isfinite(p->F_table[e])) {
+        delete c;
+        return fail(nullptr, TWMC_ERR_INVALID, "F_table has non-finite entries (fill the gaps first)");
+      }
+    c->ft_n = (int)p->F_n_times;
+  }
+  c->p.F_times = c->p.F_table = nullptr;
   c->p.omega.base = c->p.U.base = c->p.F.base = c->p.beta_init.base = c->p.beta_sigma.base = nullptr;
   c->p.omega.amp = c->p.U.amp = c->p.F.amp = c->p.beta_init.amp = c->p.beta_sigma.amp = nullptr;
   c->noisy_params = (c->noise[0] || c->noise[1] || c->noise[2]) ? 1 : 0;
@@ -364,6 +383,20 @@ int twmc_create(const twmc_params* p, int device, void** ctx_out) {
     if (ee == hipSuccess) (void)up(c->d_cscv, val, false);
     if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "upload coupling");
   }
+  if (fail_rc == TWMC_OK && c->ft_n > 0) {
+    // F(t) values and per-interval slopes (NoisyMatrix.cpp:228-239); the
+    // slope after the last time is zero, so its value is held
+    const size_t nt = (size_t)c->ft_n, n2 = 2 * (size_t)c->n;
+    std::vector<double> vals(p->F_table, p->F_table + nt * n2), slopes(nt * n2, 0.0);
+    for (size_t k = 0; k + 1 < nt; ++k)
+      for (size_t e = 0; e < n2; ++e)
+        slopes[k * n2 + e] = (vals[(k + 1) * n2 + e] - vals[k * n2 + e]) / (p->F_times[k + 1] - p->F_times[k]);
+    hipError_t ee = upload(c->d_ft_times, p->F_times, nt * sizeof(double), s);
+    if (ee == hipSuccess) ee = up(c->d_ft_vals, vals, true);
+    if (ee == hipSuccess) ee = up(c->d_ft_slopes, slopes, true);
+    if (ee == hipSuccess) ee = hipStreamSynchronize(s);
+    if (ee != hipSuccess && fail_rc == TWMC_OK) fail_rc = hip_fail(c, ee, "upload F(t) table");
+  }
   if (fail_rc == TWMC_OK && c->kernel == TWMC_KERNEL_MFMA) {
     // A = C^T, zero-padded to np x np, split into BF16 high and low parts
     c->np = mfma_padded(c->n);
@@ -412,7 +445,7 @@ void twmc_destroy(void* ctx) {
     (void)hipEventDestroy(pr.second);
   }
   DevBuf* bufs[] = {&c->d_twx, &c->d_twy, &c->d_kcx, &c->d_kcy, &c->d_cscp, &c->d_cscr,
-                    &c->d_cscv, &c->d_perm_site, &c->d_perm_k, &c->d_ahi, &c->d_alo, &c->d_kamp, &c->d_ids, &c->d_beta, &c->d_rsl,
+                    &c->d_cscv, &c->d_perm_site, &c->d_perm_k, &c->d_ahi, &c->d_alo, &c->d_kamp, &c->d_ft_times, &c->d_ft_vals, &c->d_ft_slopes, &c->d_ids, &c->d_beta, &c->d_rsl,
                     &c->d_U, &c->d_F, &c->d_partials, &c->d_obs, &c->d_m2, &c->d_m4, &c->d_traj,
                     &c->d_prev};
   for (DevBuf* b : bufs) b->release();
@@ -616,6 +649,11 @@ int twmc_batch_advance(void* ctx, uint64_t n_segments) {
       a.a_hi = c->d_ahi.p;
       a.a_lo = c->d_alo.p;
       a.np = c->np;
+      a.ft_n = c->ft_n;
+      a.t0 = c->desc.t_start + double(c->steps_done) * c->p.dt;
+      a.ft_times = static_cast<const double*>(c->d_ft_times.p);
+      a.ft_vals = c->d_ft_vals.p;
+      a.ft_slopes = c->d_ft_slopes.p;
       if (a.n_steps > 0 || a.capture) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (c->timing) {

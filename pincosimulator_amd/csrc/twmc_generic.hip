@@ -91,6 +91,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
     }
 
     const int iters = g.n_steps > 0 ? g.n_steps : (g.capture ? 1 : 0);
+    int ft_id = 0;
     for (int s = 0; s < iters; ++s) {
       // stage beta in LDS
 #pragma unroll
@@ -187,6 +188,15 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
 
       // drift + increment + update (TWMCBaseSolver.cpp:245-253)
       const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
+      // F(t) before this step (TWMCLiebSolver.cpp:252-256)
+      T fdt = T(0);
+      const cpx<T>* fv = Fp;
+      const cpx<T>* fs = nullptr;
+      if (g.ft_n > 0) {
+        fdt = T(drive_interval(g.ft_times, g.ft_n, g.t0 + double(s) * g.dt, ft_id));
+        fv = reinterpret_cast<const cpx<T>*>(g.ft_vals) + (size_t)ft_id * n;
+        fs = reinterpret_cast<const cpx<T>*>(g.ft_slopes) + (size_t)ft_id * n;
+      }
 #pragma unroll
       for (int j = 0; j < PPT; ++j) {
         int p = threadIdx.x + kBlock * j;
@@ -208,7 +218,8 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
                   hh = mk(hh.re * T(g.inv_n), hh.im * T(g.inv_n));
                 }
               }
-              cpx<T> r = rsl[i], u = Up[i], f = Fp[i];
+              cpx<T> r = rsl[i], u = Up[i], f = fv[i];
+              if (fs) f = mk(f.re + fdt * fs[i].re, f.im + fdt * fs[i].im);
               T q = z.re * z.re + z.im * z.im - T(1);
               cpx<T> c = mk(r.re + (-u.im) * q, r.im + u.re * q);
               cpx<T> cb = cmul(c, z);

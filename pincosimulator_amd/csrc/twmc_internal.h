@@ -56,7 +56,27 @@ struct StepArgs {
   const void* a_hi;
   const void* a_lo;
   int np;                    // n padded to a multiple of 32
+  // time-dependent drive: F(t) = ft_vals[id] + (t - ft_times[id]) ft_slopes[id]
+  // with id the last table time below t (F = ft_vals[0] up to ft_times[0]);
+  // t = t0 + s dt before step s of this launch, in FP64
+  int ft_n;                  // table times, 0: static F
+  double t0;
+  const double* ft_times;    // [ft_n]
+  const void* ft_vals;       // [ft_n][n] complex (T)
+  const void* ft_slopes;     // [ft_n][n] complex (T), last row zero
 };
+
+// Interval of the F(t) table used at time t and the offset t - times[id]
+// (NoisyMatrix::GetAtTimeWithSuggestion, NoisyMatrix.cpp:252-264).  `id`
+// carries the previous interval: time only grows within a launch.
+__device__ __forceinline__ double drive_interval(const double* times, int nt, double t, int& id) {
+  if (t <= times[0]) {
+    id = 0;
+    return 0.0;
+  }
+  while (id + 1 < nt && times[id + 1] < t) ++id;
+  return t - times[id];
+}
 
 struct InitArgs {
   int n, n_traj, mode;       // mode: 0 settings, 1 fixed point, 2 previous

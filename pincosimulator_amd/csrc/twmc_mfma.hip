@@ -92,6 +92,7 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
       }
 
     const int iters = CAPTURE_ONLY ? 1 : g.n_steps;
+    int ft_id = 0;
     for (int s = 0; s < iters; ++s) {
       // stage the state as the BF16 hi/lo B operand: column col (Re) and 32+col (Im)
 #pragma unroll
@@ -173,6 +174,16 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
         __syncthreads();
       }
       if (CAPTURE_ONLY) break;
+
+      if (g.ft_n > 0) {
+        // F(t) before this step into the site table (TWMCLiebSolver.cpp:252-256);
+        // the epilogue reads it after the barrier that follows the GEMM, and
+        // the previous epilogue finished before the staging barrier above
+        const float fdt = float(drive_interval(g.ft_times, g.ft_n, g.t0 + double(s) * g.dt, ft_id));
+        const cpx<float>* fv = reinterpret_cast<const cpx<float>*>(g.ft_vals) + (size_t)ft_id * n;
+        const cpx<float>* fs = reinterpret_cast<const cpx<float>*>(g.ft_slopes) + (size_t)ft_id * n;
+        for (int i = tid; i < n; i += blockDim.x) PF[i] = mk(fv[i].re + fdt * fs[i].re, fv[i].im + fdt * fs[i].im);
+      }
 
       f32x16 are[TPW], aim[TPW];
 #pragma unroll

@@ -113,6 +113,7 @@ class System:
     coupling: np.ndarray | None = None
     detuning: float = 0.0
     onsite: str = "omega"     # 'omega' (Base) or 'detuning-E' (Lieb: omega holds E)
+    F_t: tuple | None = None  # (times, table n_times x N): time-dependent drive, replaces F
     _swapped: bool = field(default=False, repr=False)
 
     def __post_init__(self):
@@ -180,6 +181,13 @@ class Engine:
             p.coupling = C.POINTER(C.c_double)()
         p.seed = int(system.seed) & 0xFFFFFFFFFFFFFFFF
         p.frame_steps = int(system.frame_steps)
+        if system.F_t is not None:
+            times = np.ascontiguousarray(np.asarray(system.F_t[0], dtype=np.float64).reshape(-1))
+            table = np.ascontiguousarray(np.asarray(system.F_t[1], dtype=np.complex128).reshape(times.size, n))
+            keep += [times, table]
+            p.F_n_times = times.size
+            p.F_times = times.ctypes.data_as(C.POINTER(C.c_double))
+            p.F_table = table.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
         ctx = C.c_void_p()
         check(L.twmc_create(C.byref(p), device, C.byref(ctx)))
         self._ctx = ctx

@@ -110,8 +110,6 @@ void TWMCHipSolver::Setup() {
     twmc_destroy(ctx_);
     ctx_ = nullptr;
   }
-  if (d.F->HasTimeDependence())
-    throw std::runtime_error("time-dependent F is not supported by the GPU solvers");
   twmc_params p{};
   std::vector<std::vector<double>> storage;
   p.nx = (int)d.nx;
@@ -127,6 +125,14 @@ void TWMCHipSolver::Setup() {
   to_matrix(*d.F, p.F, storage);
   to_matrix(*d.beta_init_val, p.beta_init, storage);
   to_matrix(*d.beta_init_sigma_val, p.beta_sigma, storage);
+  std::vector<double> ft_times, ft_values;
+  if (d.F->HasTimeDependence()) {
+    // F(t) before every step (TWMCLiebSolver.cpp:252-256)
+    d.F->TimeTable(ft_times, ft_values);
+    p.F_n_times = ft_times.size();
+    p.F_times = ft_times.data();
+    p.F_table = ft_values.data();
+  }
   check(twmc_create(&p, gpu_, &ctx_), nullptr, "twmc_create");
   nTasksToRequest = d.gpu_batch;
 }
@@ -177,7 +183,7 @@ std::vector<std::unique_ptr<TaskResults>> TWMCHipSolver::Compute(
     desc.t_start = first->t_start;
     desc.t_end = first->t_end;
     const bool want_traj = d.store != "observables";
-    desc.outputs = TWMC_OUT_OBS | (want_traj ? TWMC_OUT_TRAJ : 0u);
+    desc.outputs = (uint32_t)TWMC_OUT_OBS | (want_traj ? (uint32_t)TWMC_OUT_TRAJ : 0u);
     check(twmc_batch_begin(ctx_, &desc), ctx_, "twmc_batch_begin");
     twmc_batch_info info{};
     check(twmc_batch_info_get(ctx_, &info), ctx_, "twmc_batch_info_get");

@@ -140,6 +140,20 @@ void NoisyMatrix::SetTemporalValue(const std::map<double, std::vector<double>>& 
   timeDep_ = nt > 0;
 }
 
+void NoisyMatrix::TimeTable(std::vector<double>& times, std::vector<double>& values) const {
+  times.clear();
+  values.clear();
+  if (!timeDep_) return;
+  const size_t nt = times_.size() - 1;  // the last entry is the hold sentinel
+  for (size_t p = 0; p < nt; ++p) {
+    times.push_back(times_[p]);
+    for (const cd& v : mats_[p]) {
+      values.push_back(v.real());
+      values.push_back(v.imag());
+    }
+  }
+}
+
 std::tuple<size_t, std::vector<cd>> NoisyMatrix::GetAtTimeWithSuggestion(double t, size_t suggested) const {
   if (!timeDep_) return std::make_tuple(size_t(0), base_);
   if (t <= times_[0]) return std::make_tuple(size_t(0), mats_[0]);

@@ -54,6 +54,9 @@ class NoisyMatrix {
   std::tuple<size_t, std::vector<cd>> GetAtTimeWithSuggestion(double t, size_t suggested = 0) const;
   std::vector<cd> GetAtTime(double t) const { return std::get<1>(GetAtTimeWithSuggestion(t)); }
   const std::vector<double>& Times() const { return times_; }
+  // the filled table without the hold-after-last sentinel: times (n) and
+  // values (n x size() complex, interleaved re/im)
+  void TimeTable(std::vector<double>& times, std::vector<double>& values) const;
   bool IsUniform() const;
 
  private:

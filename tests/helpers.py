@@ -22,7 +22,7 @@ def c2_spec(**kw):
 
 
 def system(spec):
-    keys = ("nx", "ny", "dt", "gamma", "J", "omega", "U", "F", "beta_init", "beta_sigma", "frame_steps",
+    keys = ("nx", "ny", "dt", "gamma", "J", "omega", "U", "F", "F_t", "beta_init", "beta_sigma", "frame_steps",
             "seed", "t_start", "t_end", "lattice", "coupling", "detuning", "onsite")
     return System(**{k: spec[k] for k in keys if k in spec})
 

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
+from helpers import c1_spec
 from pincosimulator_amd.engine import frame_schedule
 
 
@@ -191,3 +192,47 @@ def test_truncated_loop_reports_fewer_frames():
     frames, _, tf = oracle.run_trajectory(p, 0)
     assert frames.shape[0] == 3  # 4 expected, the loop stops after 101 steps
     assert tf > 1.2345
+
+
+def _ft_spec(**kw):
+    s = dict(nx=1, ny=1, lattice=oracle.LAT_FFT, dt=0.01, gamma=0.0, J=0.0, omega=0.7, U=0.0, F=0.0,
+             beta_init=0.3 + 0.1j, beta_sigma=0.0, frame_steps=1, seed=5, t_start=0.0, t_end=3.0)
+    s.update(kw)
+    return s
+
+
+def test_time_dependent_drive_single_site_recursion():
+    """gamma = 0 (no noise), U = 0, one site: the Euler map with F(t) is
+    b_{n+1} = b_n + dt (-i omega b_n + i F(t_n)), F piecewise linear with
+    constant ends -- np.interp is an independent implementation of that rule."""
+    times = np.array([0.5, 1.0, 2.2])
+    vals = np.array([1.0 + 0.5j, -2.0 + 0.0j, 0.25 - 1.0j])
+    spec = _ft_spec(F_t=(times, vals.reshape(3, 1)))
+    fr, bf, tf = oracle.run_trajectory(spec, 0)
+    b = complex(0.3, 0.1)
+    t = 0.0
+    ref = [b]
+    while t <= 3.0 and len(ref) < fr.shape[0]:
+        F = np.interp(t, times, vals.real) + 1j * np.interp(t, times, vals.imag)
+        b = b + 0.01 * (-1j * 0.7 * b + 1j * F)
+        t += 0.01
+        ref.append(b)
+    np.testing.assert_allclose(fr[:, 0], np.array(ref[:fr.shape[0]]), rtol=0, atol=1e-13)
+
+
+def test_constant_drive_table_equals_static_drive():
+    spec = dict(c1_spec(t_end=2.0, frame_steps=50), lattice=oracle.LAT_FFT)
+    F = 1.0  # c1 drive
+    a, _, _ = oracle.run_trajectory(spec, 3)
+    b, _, _ = oracle.run_trajectory(dict(spec, F_t=(np.array([0.0]), np.full((1, 64), F, complex))), 3)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_lieb_lattice_equals_dense_lieb_coupling():
+    cells = 6
+    base = dict(nx=1, dt=0.005, gamma=1.0, J=0.0, J_AB=1.5, J_BC=0.7, detuning=0.4, E=0.2, U=0.1, F=1.2,
+                beta_init=0.0, beta_sigma=0.5, frame_steps=20, seed=9, t_start=0.0, t_end=0.4, pbc=1)
+    a, _, _ = oracle.run_trajectory(dict(base, ny=cells, lattice=oracle.LAT_LIEB), 2)
+    C = oracle.lieb_coupling(cells, 1.5, 0.7, True)
+    b, _, _ = oracle.run_trajectory(dict(base, ny=3 * cells, lattice=oracle.LAT_DENSE, coupling=C), 2)
+    np.testing.assert_array_equal(a, b)
