@@ -85,6 +85,7 @@ struct Ctx {
   size_t next_seg = 0;
   uint64_t steps_done = 0;
   int grid = 0, block = 256;
+  int cap_grid = 0;  // stencil64: grid of the capture launch (the 64x64 FFT kernel's)
   size_t lds = 0;
   // timing
   bool timing = false;
@@ -210,9 +211,20 @@ int choose_kernel(Ctx* c) {
   bool generic_ok = generic_pairs_per_thread(c->n, 256) > 0 && lds <= 160 * 1024 &&
                     (p.hopping == TWMC_HOP_MATRIX || (c->px.npass >= 0 && c->py.npass >= 0));
   bool mfma_ok = mfma_supported(c->n, p.hopping, c->precision, c->noisy_params ? 1 : 0);
+  const bool stencil_ok = stencil64_supported(p.nx, p.ny, p.hopping, c->precision,
+                                             (c->noisy_params || !uniform) ? 1 : 0) && c->ft_n == 0;
   int k = p.kernel;
+  if (k == TWMC_KERNEL_STENCIL64 && !stencil_ok)
+    return fail(c, TWMC_ERR_UNSUPPORTED,
+                "stencil 64x64 kernel needs nx = ny = 64, FFT hopping, FP32 and site-uniform, disorder-free parameters");
+  // AUTO: the 64x64 nearest-neighbour form first (the same linear map as the
+  // reference's FFT hopping, about twice as fast as the register FFT), then
+  // the register FFT, the MFMA matrix product, the LDS kernel
   if (k == TWMC_KERNEL_AUTO)
-    k = fused_ok ? TWMC_KERNEL_FUSED64 : mfma_ok ? TWMC_KERNEL_MFMA : TWMC_KERNEL_GENERIC;
+    k = stencil_ok ? TWMC_KERNEL_STENCIL64
+        : fused_ok ? TWMC_KERNEL_FUSED64
+        : mfma_ok  ? TWMC_KERNEL_MFMA
+                   : TWMC_KERNEL_GENERIC;
   if (k == TWMC_KERNEL_FUSED64 && !fused_ok)
     return fail(c, TWMC_ERR_UNSUPPORTED,
                 "fused 64x64 kernel needs nx = ny = 64, FFT hopping, FP32 and site-uniform, disorder-free parameters");
@@ -222,6 +234,51 @@ int choose_kernel(Ctx* c) {
     return fail(c, TWMC_ERR_UNSUPPORTED,
                 "MFMA kernel needs matrix hopping, FP32, at most 512 sites and no per-trajectory disorder");
   c->kernel = k;
+  return TWMC_OK;
+}
+
+hipError_t timed(Ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>& evs, hipStream_t s, bool begin,
+                 hipEvent_t& e) {
+  if (!c->timing) return hipSuccess;
+  hipError_t r = hipSuccess;
+  if (begin) {
+    r = hipEventCreate(&e);
+    if (r == hipSuccess) r = hipEventRecord(e, s);
+  } else {
+    hipEvent_t e1 = nullptr;
+    r = hipEventCreate(&e1);
+    if (r == hipSuccess) r = hipEventRecord(e1, s);
+    if (r == hipSuccess) evs.push_back({e, e1});
+  }
+  return r;
+}
+
+// stencil64 segment: the capture (if any) is a launch of the 64x64 FFT
+// kernel's capture-only variant plus the reduction, then the steps
+int advance_stencil(Ctx* c, const StepArgs& a, hipStream_t s) {
+  const int n = c->n;
+  if (a.capture) {
+    StepArgs ca = a;
+    ca.n_steps = 0;
+    hipEvent_t r0 = nullptr;
+    HIPCHK(c, timed(c, c->ev_red, s, true, r0));
+    HIPCHK(c, launch_fused64(ca, c->cap_grid, s));
+    ReduceArgs r{};
+    r.n = n;
+    r.slots = c->cap_grid;
+    r.partials_in = static_cast<const double*>(c->d_partials.p);
+    r.partials = static_cast<double*>(c->d_partials.p);
+    r.obs = static_cast<double*>(c->d_obs.p) + (size_t)a.frame * kQ * n;
+    HIPCHK(c, launch_reduce(r, s));
+    HIPCHK(c, timed(c, c->ev_red, s, false, r0));
+  }
+  if (a.n_steps > 0) {
+    hipEvent_t e0 = nullptr;
+    HIPCHK(c, timed(c, c->ev_step, s, true, e0));
+    HIPCHK(c, launch_stencil64(a, c->grid, s));
+    HIPCHK(c, timed(c, c->ev_step, s, false, e0));
+    c->stats.site_steps += (uint64_t)a.n_steps * (uint64_t)n * c->n_traj;
+  }
   return TWMC_OK;
 }
 
@@ -502,6 +559,14 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
     c->block = 64;
     c->lds = 0;
     HIPCHK(c, fused64_occupancy(&per_cu));
+  } else if (c->kernel == TWMC_KERNEL_STENCIL64) {
+    c->block = 64;
+    c->lds = 0;
+    int cap_per_cu = 0;
+    HIPCHK(c, fused64_occupancy(&cap_per_cu));
+    if (cap_per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "capture kernel does not fit on a CU");
+    c->cap_grid = (int)std::min<size_t>(B, (size_t)cap_per_cu * cus);
+    HIPCHK(c, stencil64_occupancy(&per_cu));
   } else if (c->kernel == TWMC_KERNEL_MFMA) {
     c->block = 256;
     c->lds = mfma_lds_bytes(c->np);
@@ -521,7 +586,7 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
   HIPCHK(c, c->d_rsl.ensure(sets * n * c->es));
   HIPCHK(c, c->d_U.ensure(sets * n * c->es));
   HIPCHK(c, c->d_F.ensure(sets * n * c->es));
-  size_t part_bytes = (size_t)c->grid * kQ * n * sizeof(double);
+  size_t part_bytes = (size_t)std::max(c->grid, c->cap_grid) * kQ * n * sizeof(double);
   HIPCHK(c, c->d_partials.ensure(part_bytes));
   HIPCHK(c, hipMemsetAsync(c->d_partials.p, 0, part_bytes, s));
   size_t nf = std::max<uint64_t>(c->n_frames, 1);
@@ -654,7 +719,10 @@ int twmc_batch_advance(void* ctx, uint64_t n_segments) {
       a.ft_times = static_cast<const double*>(c->d_ft_times.p);
       a.ft_vals = c->d_ft_vals.p;
       a.ft_slopes = c->d_ft_slopes.p;
-      if (a.n_steps > 0 || a.capture) {
+      if (c->kernel == TWMC_KERNEL_STENCIL64) {
+        int rc = advance_stencil(c, a, s);
+        if (rc != TWMC_OK) return rc;
+      } else if (a.n_steps > 0 || a.capture) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (c->timing) {
           HIPCHK(c, hipEventCreate(&e0));

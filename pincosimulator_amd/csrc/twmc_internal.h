@@ -134,6 +134,9 @@ bool fused64_supported(int nx, int ny, int hopping, int precision, int param_str
 hipError_t launch_fused64(const StepArgs& a, int grid, hipStream_t s);
 hipError_t fused64_occupancy(int* blocks_per_cu);
 void fused64_perms(int* perm_site, int* perm_k);  // [4096] each
+bool stencil64_supported(int nx, int ny, int hopping, int precision, int param_stride);
+hipError_t launch_stencil64(const StepArgs& a, int grid, hipStream_t s);
+hipError_t stencil64_occupancy(int* blocks_per_cu);
 
 // matrix hopping on MFMA (FP32 state, BF16 x 3 products, FP32 accumulation)
 bool mfma_supported(int n, int hopping, int precision, int param_stride);

@@ -1,0 +1,112 @@
+// twmc_stencil64.hip -- register-resident TWMC step kernel for a 64 x 64
+// periodic lattice in FP32 with the hopping applied as its exact real-space
+// form.
+//
+// The reference applies the hopping in k-space (TWMCBaseSolver.cpp:80-88,
+// :235-242): h = IDFT(K . DFT(beta)) / N with
+// K = -2iJ([nx != 1] cos(2 pi kx / nx) + [ny != 1] cos(2 pi ky / ny)).
+// Because e^{ik} + e^{-ik} = 2 cos k, that product is, term by term, the
+// nearest-neighbour sum
+//     h(x, y) = -iJ (beta(x-1, y) + beta(x+1, y) + beta(x, y-1) + beta(x, y+1))
+// with periodic wrap -- the same linear map, without the FFT round trip (the
+// two agree to rounding, ~1e-16 relative in FP64).  This kernel evaluates that
+// form: one wavefront owns one trajectory, lane x holds row x (64 complex
+// values in registers); y-neighbours are neighbouring registers, x-neighbours
+// come from lanes x-1 and x+1 (wave-wide rotations).  Per step and site: the
+// 4-term sum, the local drift, one Philox4x32-10 call per site pair with
+// Box-Muller, the update.  Frame captures run as a separate launch of the
+// 64 x 64 FFT kernel's capture-only variant (it also forms the k-space sums),
+// so this loop carries no capture code and no LDS at all.
+#include "twmc_cplx.h"
+#include "twmc_internal.h"
+#include "twmc_rng.h"
+
+namespace twmc {
+
+namespace {
+
+constexpr int kL = 64;
+
+__device__ __forceinline__ float lane_from(float v, int src) { return __shfl(v, src, 64); }
+
+__global__ void __launch_bounds__(64, 2) stencil64_step_kernel(StepArgs g) {
+  const int lane = threadIdx.x;  // row x
+  const int xm = (lane + 63) & 63, xp = (lane + 1) & 63;
+
+  // uniform site parameters (the kernel is chosen only when they are uniform)
+  const cpx<float> r0 = reinterpret_cast<const cpx<float>*>(g.rsl)[0];
+  const cpx<float> u0 = reinterpret_cast<const cpx<float>*>(g.U)[0];
+  const cpx<float> f0 = reinterpret_cast<const cpx<float>*>(g.F)[0];
+  const float dtf = float(g.dt);
+  const float A = dtf * r0.re, Bc = dtf * r0.im;    // dt * rsl
+  const float Ur = dtf * u0.re, Ui = dtf * u0.im;   // dt * U
+  const float fr = -dtf * f0.im, fi = dtf * f0.re;  // dt * i F
+  const float sig = float(g.dt4);
+  // dt h = -i J dt S: the FFT form's -2J (kscale) times the 1/2 of each cosine
+  const float jdt = float(-0.5 * g.kscale * g.dt);
+
+  for (int b = blockIdx.x; b < g.n_traj; b += gridDim.x) {
+    const uint64_t traj = g.traj_ids[b];
+    cpx<float>* gb = reinterpret_cast<cpx<float>*>(g.beta) + (size_t)b * 4096 + (size_t)lane * kL;
+    cpx<float> beta[kL];
+    // lane x reads its own row: 512 contiguous bytes, 16-byte loads
+#pragma unroll
+    for (int y = 0; y < kL; y += 2) {
+      const float4 v = *reinterpret_cast<const float4*>(gb + y);
+      beta[y] = mk(v.x, v.y);
+      beta[y + 1] = mk(v.z, v.w);
+    }
+
+    for (int s = 0; s < g.n_steps; ++s) {
+      // opaque per step so the compiler keeps the Philox rounds inside the loop
+      uint32_t pbase = (uint32_t)lane * 32u;
+      asm volatile("" : "+v"(pbase));
+      const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
+      const cpx<float> first = beta[0];  // old value of the y + 1 wrap of y = 63
+      cpx<float> prev = beta[kL - 1];     // old value of y - 1
+#pragma unroll
+      for (int p = 0; p < kL / 2; ++p) {
+        const u32x4 w = philox4x32_10(make_counter(traj, TAG_WIENER, step, pbase + (uint32_t)p), g.key0, g.key1);
+        float xr[2], xi[2];
+        box_muller_f32(w.x, w.y, xr[0], xi[0]);
+        box_muller_f32(w.z, w.w, xr[1], xi[1]);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int y = 2 * p + h;
+          const cpx<float> v = beta[y];
+          const cpx<float> yn = (y == kL - 1) ? first : beta[y + 1];
+          // every lane reads its neighbours' old beta[y] before any lane writes it
+          const float sr = lane_from(v.re, xm) + lane_from(v.re, xp) + prev.re + yn.re;
+          const float si = lane_from(v.im, xm) + lane_from(v.im, xp) + prev.im + yn.im;
+          prev = v;
+          const float q = v.re * v.re + v.im * v.im - 1.0f;
+          const float cr = A - Ui * q, ci = Bc + Ur * q;  // dt (rsl + i U (|b|^2 - 1))
+          const float nr = v.re + (cr * v.re - ci * v.im) + fmaf(jdt, si, fr) + sig * xr[h];
+          const float ni = v.im + (cr * v.im + ci * v.re) + fmaf(-jdt, sr, fi) + sig * xi[h];
+          beta[y] = mk(nr, ni);
+        }
+      }
+    }
+
+#pragma unroll
+    for (int y = 0; y < kL; y += 2)
+      *reinterpret_cast<float4*>(gb + y) = make_float4(beta[y].re, beta[y].im, beta[y + 1].re, beta[y + 1].im);
+  }
+}
+
+}  // namespace
+
+bool stencil64_supported(int nx, int ny, int hopping, int precision, int param_stride) {
+  return nx == 64 && ny == 64 && hopping == 0 && precision == 32 && param_stride == 0;
+}
+
+hipError_t launch_stencil64(const StepArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(stencil64_step_kernel, dim3(grid), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t stencil64_occupancy(int* blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)stencil64_step_kernel, 64, 0);
+}
+
+}  // namespace twmc

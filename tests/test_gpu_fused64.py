@@ -12,10 +12,13 @@ from pincosimulator_amd.observables import from_trajectories
 pytestmark = pytest.mark.gpu
 
 
-def test_fused_selected_for_64x64_fp32():
+def test_kernel_selection_for_64x64_fp32():
+    """AUTO takes the nearest-neighbour form; the register FFT is selectable."""
     eng = Engine(system(c2_spec(t_end=0.002 * 10, frame_steps=10)), precision=32)
-    res = eng.run(np.arange(4))
-    assert res.kernel == "fused64"
+    assert eng.run(np.arange(4)).kernel == "stencil64"
+    eng.close()
+    eng = Engine(system(c2_spec(t_end=0.002 * 10, frame_steps=10)), precision=32, kernel="fused64")
+    assert eng.run(np.arange(4)).kernel == "fused64"
     eng.close()
 
 
@@ -24,7 +27,7 @@ def test_fused_vs_fp64_oracle_per_trajectory():
     per-trajectory error, bounded at 2e-4 of max |beta| (measured ~1e-5)."""
     spec = c2_spec(t_end=2.0, frame_steps=500)
     ids = np.array([0, 5, 16383])
-    eng = Engine(system(spec), precision=32)
+    eng = Engine(system(spec), precision=32, kernel="fused64")
     res = eng.run(ids, traj=True, final=True)
     assert res.kernel == "fused64" and res.n_steps == 1000
     for j, t in enumerate(ids):
@@ -60,7 +63,7 @@ def test_fused_observables_vs_fp64():
     agreement far inside 3 standard errors; the error is printed per observable."""
     spec = c2_spec(t_end=4.0, frame_steps=500)
     ids = np.arange(256)
-    r32 = Engine(system(spec), precision=32).run(ids)
+    r32 = Engine(system(spec), precision=32, kernel="fused64").run(ids)
     r64 = Engine(system(spec), precision=64, kernel="generic").run(ids)
     n32, se32 = observables.mean_occupation(r32)
     n64, se64 = observables.mean_occupation(r64)

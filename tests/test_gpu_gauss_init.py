@@ -41,16 +41,17 @@ def test_gaussian_start_fp64_matches_oracle(nx, ny):
         assert rel_err(res.traj[j], fr) < 1e-9
 
 
-def test_gaussian_start_fp32_fused64():
+@pytest.mark.parametrize("kernel", ["stencil64", "fused64"])
+def test_gaussian_start_fp32_64x64(kernel):
     spec = c2_spec(t_end=0.2, frame_steps=50)
     mean = 1.2 + 0.5j
     nk = nk_profile(64, 64)
     ids = np.arange(64) + 5000
-    print("gaussian start fp32 64x64", flush=True)
-    eng = Engine(system(spec), precision=32, kernel="auto")
+    print(f"gaussian start fp32 64x64 {kernel}", flush=True)
+    eng = Engine(system(spec), precision=32, kernel=kernel)
     res = eng.run(ids, traj=True, init="gaussian", gauss_mean=mean, gauss_nk=nk)
     eng.close()
-    assert res.kernel == "fused64"
+    assert res.kernel == kernel
     for j in (0, 17, 63):
         b0 = oracle.gaussian_state(64, 64, spec["seed"], int(ids[j]), mean, nk).reshape(-1)
         assert rel_err(res.traj[j, 0], b0) < 2e-6

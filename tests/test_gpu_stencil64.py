@@ -1,0 +1,70 @@
+"""64x64 FP32 kernel with the hopping in its exact nearest-neighbour form,
+checked against the FP64 oracle (which runs the reference's FFT form) per
+trajectory, against the FFT kernel, and through its ensemble sums (captured by
+the FFT kernel's capture launch)."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import c2_spec, rel_err, system
+from pincosimulator_amd import Engine, observables
+from pincosimulator_amd.observables import from_trajectories
+
+pytestmark = pytest.mark.gpu
+
+
+def test_stencil_vs_fp64_oracle_per_trajectory():
+    """FP32 vs FP64 oracle, same Philox stream, 1000 steps: bounded at 2e-4 of
+    max |beta|, as the FFT kernel."""
+    spec = c2_spec(t_end=2.0, frame_steps=500)
+    ids = np.array([0, 5, 16383])
+    print("stencil64 vs oracle", flush=True)
+    eng = Engine(system(spec), precision=32, kernel="stencil64")
+    res = eng.run(ids, traj=True, final=True)
+    assert res.kernel == "stencil64" and res.n_steps == 1000
+    for j, t in enumerate(ids):
+        fr, bf, _ = oracle.run_trajectory(dict(spec, lattice=oracle.LAT_FFT), int(t))
+        err = rel_err(res.traj[j], fr)
+        print(f"traj {t}: FP32 stencil vs FP64 oracle max rel err over 3 frames = {err:.3e}")
+        assert err < 2e-4
+        assert rel_err(res.beta_final[j], bf) < 2e-4
+    eng.close()
+
+
+def test_stencil_vs_fft_kernel_fp32():
+    spec = c2_spec(t_end=0.2, frame_steps=50)
+    ids = np.arange(8)
+    a = Engine(system(spec), precision=32, kernel="stencil64").run(ids, traj=True)
+    b = Engine(system(spec), precision=32, kernel="fused64").run(ids, traj=True)
+    assert rel_err(a.traj, b.traj) < 2e-5
+
+
+def test_stencil_ensemble_sums_and_many_trajectories():
+    """More trajectories than resident waves (the persistent loop), sums vs
+    the stored frames."""
+    spec = c2_spec(t_end=0.1, frame_steps=25, U=0.2, F=1.0)
+    eng = Engine(system(spec), precision=32, kernel="stencil64")
+    res = eng.run(np.arange(50), traj=True)
+    ref = from_trajectories(res.traj, (64, 64))
+    for k in ("sum_n", "sum_n2", "sum_beta", "traj_m2", "traj_m4"):
+        assert rel_err(getattr(res, k), ref[k]) < 1e-9, k
+    assert rel_err(res.sum_nk, ref["sum_nk"]) < 1e-5
+    eng.close()
+    big = Engine(system(c2_spec(t_end=0.02, frame_steps=10)), precision=32, kernel="stencil64")
+    r = big.run(np.arange(5000))
+    n, se = observables.mean_occupation(r)
+    assert np.all(np.isfinite(r.sum_n)) and n.shape == (2,)
+    big.close()
+
+
+def test_stencil_observables_vs_fp64():
+    spec = c2_spec(t_end=4.0, frame_steps=500)
+    ids = np.arange(256)
+    r32 = Engine(system(spec), precision=32, kernel="stencil64").run(ids)
+    r64 = Engine(system(spec), precision=64, kernel="generic").run(ids)
+    n32, _ = observables.mean_occupation(r32)
+    n64, se64 = observables.mean_occupation(r64)
+    g32, gse = observables.g2_jackknife(r32)
+    g64, _ = observables.g2_jackknife(r64)
+    assert np.all(np.abs(n32 - n64) < 3 * se64 + 1e-12)
+    assert np.all(np.abs(g32[1:] - g64[1:]) < 3 * gse[1:])
