@@ -130,6 +130,21 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
       }
     }
 
+    // row and column of each owned pair's even site (ny is even, so the odd
+    // site is the next column of the same row)
+    int prow[kPPT], pcol[kPPT];
+#pragma unroll
+    for (int j = 0; j < kPPT; ++j) {
+      const int p = threadIdx.x + kSweepBlock * j;
+      prow[j] = (2 * p) / ny;
+      pcol[j] = 2 * p - prow[j] * ny;
+    }
+    // hopping h = -i J S, S the sum over nearest neighbours of the lattice
+    // (periodic; a dimension of length 1 has none): the exact real-space form
+    // of the reference's IDFT(K DFT(beta)) / N (TWMCBaseSolver.cpp:80-88, :235-242)
+    const float jre = -0.5f * P.kscale;  // J
+    const bool hop_y = ny > 1, hop_x = P.fft_x && nx > 1;
+
     const int iters = CAPTURE_ONLY ? 1 : g.n_steps;
     for (int s = 0; s < iters; ++s) {
 #pragma unroll
@@ -220,15 +235,15 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
         }
       }
 
-      // rows (length ny), then columns (length nx) for 2D points
-      cpx<float>* cur = fft_dim<-1>(bufA, bufB, P.py, nx, ny, 1, twy);
-      if (P.fft_x) {
-        cpx<float>* other = (cur == bufA) ? bufB : bufA;
-        cur = fft_dim<-1>(cur, other, P.px, ny, 1, ny, twx);
-      }
-
       if (cap) {
-        // |DFT beta|^2 per radial bin, summed over this item's trajectories
+        // k-space populations: forward FFT, rows (length ny) then columns
+        // (length nx) for 2D points; |DFT beta|^2 per radial bin, summed over
+        // this item's trajectories
+        cpx<float>* cur = fft_dim<-1>(bufA, bufB, P.py, nx, ny, 1, twy);
+        if (P.fft_x) {
+          cpx<float>* other = (cur == bufA) ? bufB : bufA;
+          cur = fft_dim<-1>(cur, other, P.px, ny, 1, ny, twx);
+        }
         for (int i = threadIdx.x; i < m; i += blockDim.x) {
           const cpx<float> f = cur[i];
           const int mode = P.fft_x ? i : i % ny;
@@ -237,28 +252,20 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
         __syncthreads();
         if (threadIdx.x < kBins + 2)
           atomicAdd(&g.ksum[((size_t)P.index * g.n_frames + g.frame) * (kBins + 2) + threadIdx.x], s_bins[threadIdx.x]);
+        // the transform overwrote bufA: stage beta again
+#pragma unroll
+        for (int j = 0; j < kPPT; ++j) {
+          const int p = threadIdx.x + kSweepBlock * j;
+          if (p < npairs) {
+            bufA[2 * p] = beta[2 * j];
+            bufA[2 * p + 1] = beta[2 * j + 1];
+          }
+        }
         __syncthreads();
       }
       if (CAPTURE_ONLY) break;
 
-      for (int i = threadIdx.x; i < m; i += blockDim.x) {
-        const int kx = i / ny, ky = i - kx * ny;
-        const float kim = P.kscale * ((P.fft_x ? kcx[kx] : 0.0f) + kcy[ky]);
-        const cpx<float> f = cur[i];
-        cur[i] = mk(-kim * f.im, kim * f.re);
-      }
-      __syncthreads();
-      {
-        cpx<float>* other = (cur == bufA) ? bufB : bufA;
-        cur = fft_dim<+1>(cur, other, P.py, nx, ny, 1, twy);
-        if (P.fft_x) {
-          other = (cur == bufA) ? bufB : bufA;
-          cur = fft_dim<+1>(cur, other, P.px, ny, 1, ny, twx);
-        }
-      }
-
       const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
-      const float inv_n = 1.0f / float(P.fft_x ? m : ny);
       const float dt = g.dt;
 #pragma unroll
       for (int j = 0; j < kPPT; ++j) {
@@ -273,10 +280,23 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
           box_muller_f32(w.z, w.w, xr[1], xi[1]);
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
-            const int i = 2 * p + h;
+            const int col = pcol[j] + h, rb = prow[j] * ny;
             cpx<float> z = beta[2 * j + h];
-            cpx<float> hh = cur[i];
-            hh = mk(hh.re * inv_n, hh.im * inv_n);
+            float sr = 0.0f, si = 0.0f;
+            if (hop_y) {
+              const cpx<float> a = bufA[rb + (col + 1 == ny ? 0 : col + 1)];
+              const cpx<float> b = bufA[rb + (col == 0 ? ny - 1 : col - 1)];
+              sr = a.re + b.re;
+              si = a.im + b.im;
+            }
+            if (hop_x) {
+              const int r = prow[j];
+              const cpx<float> a = bufA[(r + 1 == nx ? 0 : r + 1) * ny + col];
+              const cpx<float> b = bufA[(r == 0 ? nx - 1 : r - 1) * ny + col];
+              sr += a.re + b.re;
+              si += a.im + b.im;
+            }
+            const cpx<float> hh = mk(jre * si, -jre * sr);
             const float q = z.re * z.re + z.im * z.im - 1.0f;
             const cpx<float> c = mk(P.rsl_re + (-P.u_im) * q, P.rsl_im + P.u_re * q);
             const cpx<float> cb = cmul(c, z);
