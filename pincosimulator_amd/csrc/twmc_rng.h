@@ -44,6 +44,17 @@ __host__ __device__ __forceinline__ void mulhilo32(uint32_t a, uint32_t b, uint3
   lo = uint32_t(p);
 }
 
+// a ^ b ^ c; on the GPU one v_bitop3_b32 (truth table 0x96) instead of two
+// XORs.  Used from the third round on, where every operand is per-lane; the
+// first two rounds keep plain XORs so that their uniform parts stay scalar.
+__host__ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+  return a ^ b ^ c;
+#endif
+}
+
 // Ten Philox rounds; the key is bumped by the Weyl constants between rounds.
 __host__ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -56,9 +67,9 @@ __host__ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, ui
     mulhilo32(0xD2511F53u, c.x, hi0, lo0);
     mulhilo32(0xCD9E8D57u, c.z, hi1, lo1);
     u32x4 n;
-    n.x = hi1 ^ c.y ^ k0;
+    n.x = r < 2 ? (hi1 ^ c.y ^ k0) : xor3(hi1, c.y, k0);
     n.y = lo1;
-    n.z = hi0 ^ c.w ^ k1;
+    n.z = r < 2 ? (hi0 ^ c.w ^ k1) : xor3(hi0, c.w, k1);
     n.w = lo0;
     c = n;
   }
