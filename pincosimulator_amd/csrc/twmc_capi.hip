@@ -458,11 +458,12 @@ int twmc_create(const twmc_params* p, int device, void** ctx_out) {
     // A = C^T, zero-padded to np x np, split into BF16 high and low parts
     c->np = mfma_padded(c->n);
     const int np = c->np;
-    std::vector<double> a((size_t)np * np, 0.0);
+    const int rows = mfma_rows(np);
+    std::vector<double> a((size_t)rows * np, 0.0);
     for (int i = 0; i < c->n; ++i)
       for (int j = 0; j < c->n; ++j) a[(size_t)j * np + i] = p->coupling[(size_t)i * c->n + j];
-    std::vector<uint16_t> hi((size_t)np * np), lo((size_t)np * np);
-    mfma_split(a.data(), np, hi.data(), lo.data());
+    std::vector<uint16_t> hi((size_t)rows * np), lo((size_t)rows * np);
+    mfma_split(a.data(), a.size(), hi.data(), lo.data());
     hipError_t ee = upload(c->d_ahi, hi.data(), hi.size() * 2, s);
     if (ee == hipSuccess) ee = upload(c->d_alo, lo.data(), lo.size() * 2, s);
     if (ee == hipSuccess) ee = hipStreamSynchronize(s);

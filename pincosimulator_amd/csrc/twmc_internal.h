@@ -145,7 +145,8 @@ size_t mfma_lds_bytes(int np);
 hipError_t launch_mfma(const StepArgs& a, int grid, hipStream_t s);
 hipError_t mfma_occupancy(int np, int* blocks_per_cu);
 // splits A (row-major np x np, FP64) into BF16 hi and lo parts (raw 16-bit)
-void mfma_split(const double* a, int np, uint16_t* hi, uint16_t* lo);
+void mfma_split(const double* a, size_t count, uint16_t* hi, uint16_t* lo);
+int mfma_rows(int np);  // rows of the zero-padded A operand
 
 }  // namespace twmc
 

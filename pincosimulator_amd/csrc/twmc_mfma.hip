@@ -37,12 +37,18 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kTB = 32;  // trajectories per workgroup
 
-__device__ __forceinline__ uint16_t bf16_bits(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-  return uint16_t(u >> 16);
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// two FP32 values -> packed BF16 high parts (round to nearest even, one
+// v_cvt_pk_bf16_f32) and packed BF16 parts of the remainders, so that
+// x = hi + lo to ~2^-16 relative
+__device__ __forceinline__ void split_bf16(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+  const f32x2 v = {x0, x1};
+  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+  const f32x2 d = {x0 - __uint_as_float(hi << 16), x1 - __uint_as_float(hi & 0xffff0000u)};
+  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(d, bf16x2));
 }
-__device__ __forceinline__ float bf16_float(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 
 template <int TPW, bool CAPTURE_ONLY>
 __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
@@ -101,22 +107,16 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
         for (int q = 0; q < 4; ++q) {
           if (wave * TPW + t >= tiles) continue;
           const int i0 = (wave * TPW + t) * 32 + q * 8 + half * 4;
-          uint16_t rh[4], rl[4], ih[4], il[4];
+          uint32_t rh[2], rl[2], ih[2], il[2];
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const float xr = sre[t][4 * q + e], xi = sim[t][4 * q + e];
-            rh[e] = bf16_bits(xr);
-            rl[e] = bf16_bits(xr - bf16_float(rh[e]));
-            ih[e] = bf16_bits(xi);
-            il[e] = bf16_bits(xi - bf16_float(ih[e]));
+          for (int e = 0; e < 2; ++e) {
+            split_bf16(sre[t][4 * q + 2 * e], sre[t][4 * q + 2 * e + 1], rh[e], rl[e]);
+            split_bf16(sim[t][4 * q + 2 * e], sim[t][4 * q + 2 * e + 1], ih[e], il[e]);
           }
-          auto pack = [](const uint16_t* v) {
-            return make_uint2(uint32_t(v[0]) | (uint32_t(v[1]) << 16), uint32_t(v[2]) | (uint32_t(v[3]) << 16));
-          };
-          *reinterpret_cast<uint2*>(&Xhi[col * ldk + i0]) = pack(rh);
-          *reinterpret_cast<uint2*>(&Xlo[col * ldk + i0]) = pack(rl);
-          *reinterpret_cast<uint2*>(&Xhi[(32 + col) * ldk + i0]) = pack(ih);
-          *reinterpret_cast<uint2*>(&Xlo[(32 + col) * ldk + i0]) = pack(il);
+          *reinterpret_cast<uint2*>(&Xhi[col * ldk + i0]) = make_uint2(rh[0], rh[1]);
+          *reinterpret_cast<uint2*>(&Xlo[col * ldk + i0]) = make_uint2(rl[0], rl[1]);
+          *reinterpret_cast<uint2*>(&Xhi[(32 + col) * ldk + i0]) = make_uint2(ih[0], ih[1]);
+          *reinterpret_cast<uint2*>(&Xlo[(32 + col) * ldk + i0]) = make_uint2(il[0], il[1]);
         }
       __syncthreads();
 
@@ -194,24 +194,61 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
           aim[t][r] = 0.f;
         }
       }
-      for (int kb = 0; kb < np; kb += 16) {
-        const int ko = kb + half * 8;
-        const bf16x8 brh = *reinterpret_cast<const bf16x8*>(&Xhi[col * ldk + ko]);
-        const bf16x8 brl = *reinterpret_cast<const bf16x8*>(&Xlo[col * ldk + ko]);
-        const bf16x8 bih = *reinterpret_cast<const bf16x8*>(&Xhi[(32 + col) * ldk + ko]);
-        const bf16x8 bil = *reinterpret_cast<const bf16x8*>(&Xlo[(32 + col) * ldk + ko]);
+      // A has 4 TPW 32 zero-padded rows, so every wave runs all TPW tiles
+      // without a guard; A fragments are double-buffered one 16-wide k block
+      // ahead of the MFMAs that use them (np / 16 is even)
+      const size_t abase = (size_t)(wave * TPW * 32 + col) * np + half * 8;
+      bf16x8 ah0[TPW], al0[TPW], ah1[TPW], al1[TPW];
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) {
+        ah0[t] = *reinterpret_cast<const bf16x8*>(&Ahi[abase + (size_t)t * 32 * np]);
+        al0[t] = *reinterpret_cast<const bf16x8*>(&Alo[abase + (size_t)t * 32 * np]);
+      }
+      for (int kb = 0; kb < np; kb += 32) {
+        // k block kb: buffer 0; prefetch kb + 16 into buffer 1
 #pragma unroll
         for (int t = 0; t < TPW; ++t) {
-          if (wave * TPW + t >= tiles) continue;
-          const size_t arow = (size_t)((wave * TPW + t) * 32 + col) * np + ko;
-          const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&Ahi[arow]);
-          const bf16x8 al = *reinterpret_cast<const bf16x8*>(&Alo[arow]);
-          are[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, brh, are[t], 0, 0, 0);
-          aim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bih, aim[t], 0, 0, 0);
-          are[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, brl, are[t], 0, 0, 0);
-          aim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bil, aim[t], 0, 0, 0);
-          are[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, brh, are[t], 0, 0, 0);
-          aim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bih, aim[t], 0, 0, 0);
+          ah1[t] = *reinterpret_cast<const bf16x8*>(&Ahi[abase + (size_t)t * 32 * np + kb + 16]);
+          al1[t] = *reinterpret_cast<const bf16x8*>(&Alo[abase + (size_t)t * 32 * np + kb + 16]);
+        }
+        {
+          const int ko = kb + half * 8;
+          const bf16x8 brh = *reinterpret_cast<const bf16x8*>(&Xhi[col * ldk + ko]);
+          const bf16x8 brl = *reinterpret_cast<const bf16x8*>(&Xlo[col * ldk + ko]);
+          const bf16x8 bih = *reinterpret_cast<const bf16x8*>(&Xhi[(32 + col) * ldk + ko]);
+          const bf16x8 bil = *reinterpret_cast<const bf16x8*>(&Xlo[(32 + col) * ldk + ko]);
+#pragma unroll
+          for (int t = 0; t < TPW; ++t) {
+            are[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0[t], brh, are[t], 0, 0, 0);
+            aim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0[t], bih, aim[t], 0, 0, 0);
+            are[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0[t], brl, are[t], 0, 0, 0);
+            aim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0[t], bil, aim[t], 0, 0, 0);
+            are[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0[t], brh, are[t], 0, 0, 0);
+            aim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0[t], bih, aim[t], 0, 0, 0);
+          }
+        }
+        // k block kb + 16: buffer 1; prefetch kb + 32 (the last block reloads itself)
+        const int kn = kb + 32 < np ? kb + 32 : kb + 16;
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+          ah0[t] = *reinterpret_cast<const bf16x8*>(&Ahi[abase + (size_t)t * 32 * np + kn]);
+          al0[t] = *reinterpret_cast<const bf16x8*>(&Alo[abase + (size_t)t * 32 * np + kn]);
+        }
+        {
+          const int ko = kb + 16 + half * 8;
+          const bf16x8 brh = *reinterpret_cast<const bf16x8*>(&Xhi[col * ldk + ko]);
+          const bf16x8 brl = *reinterpret_cast<const bf16x8*>(&Xlo[col * ldk + ko]);
+          const bf16x8 bih = *reinterpret_cast<const bf16x8*>(&Xhi[(32 + col) * ldk + ko]);
+          const bf16x8 bil = *reinterpret_cast<const bf16x8*>(&Xlo[(32 + col) * ldk + ko]);
+#pragma unroll
+          for (int t = 0; t < TPW; ++t) {
+            are[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1[t], brh, are[t], 0, 0, 0);
+            aim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1[t], bih, aim[t], 0, 0, 0);
+            are[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1[t], brl, are[t], 0, 0, 0);
+            aim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1[t], bil, aim[t], 0, 0, 0);
+            are[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1[t], brh, are[t], 0, 0, 0);
+            aim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1[t], bih, aim[t], 0, 0, 0);
+          }
         }
       }
       __syncthreads();  // the next step overwrites the B operand
@@ -281,6 +318,9 @@ hipError_t launch_t(const StepArgs& a, int grid, size_t lds, hipStream_t s) {
 
 int mfma_padded(int n) { return (n + 31) / 32 * 32; }
 
+// rows of the padded A operand: 4 waves x TPW tiles x 32
+int mfma_rows(int np) { return 4 * ((np / 32 + 3) / 4) * 32; }
+
 size_t mfma_lds_bytes(int np) {
   return (size_t)2 * 64 * (np + 8) * 2 + (size_t)3 * np * 8 + 4 * 32 * 2 * 8;
 }
@@ -314,7 +354,7 @@ hipError_t mfma_occupancy(int np, int* blocks_per_cu) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 256, lds);
 }
 
-void mfma_split(const double* a, int np, uint16_t* hi, uint16_t* lo) {
+void mfma_split(const double* a, size_t count, uint16_t* hi, uint16_t* lo) {
   auto rne = [](float f) {
     uint32_t u;
     std::memcpy(&u, &f, 4);
@@ -327,7 +367,7 @@ void mfma_split(const double* a, int np, uint16_t* hi, uint16_t* lo) {
     std::memcpy(&f, &u, 4);
     return f;
   };
-  for (size_t k = 0; k < (size_t)np * np; ++k) {
+  for (size_t k = 0; k < count; ++k) {
     float f = (float)a[k];
     hi[k] = rne(f);
     lo[k] = rne(f - back(hi[k]));
