@@ -462,8 +462,10 @@ int twmc_create(const twmc_params* p, int device, void** ctx_out) {
     std::vector<double> a((size_t)rows * np, 0.0);
     for (int i = 0; i < c->n; ++i)
       for (int j = 0; j < c->n; ++j) a[(size_t)j * np + i] = p->coupling[(size_t)i * c->n + j];
+    std::vector<double> as(a.size());
+    mfma_swizzle(a.data(), rows, np, as.data());
     std::vector<uint16_t> hi((size_t)rows * np), lo((size_t)rows * np);
-    mfma_split(a.data(), a.size(), hi.data(), lo.data());
+    mfma_split(as.data(), as.size(), hi.data(), lo.data());
     hipError_t ee = upload(c->d_ahi, hi.data(), hi.size() * 2, s);
     if (ee == hipSuccess) ee = upload(c->d_alo, lo.data(), lo.size() * 2, s);
     if (ee == hipSuccess) ee = hipStreamSynchronize(s);

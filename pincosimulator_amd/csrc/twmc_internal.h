@@ -147,6 +147,7 @@ hipError_t mfma_occupancy(int np, int* blocks_per_cu);
 // splits A (row-major np x np, FP64) into BF16 hi and lo parts (raw 16-bit)
 void mfma_split(const double* a, size_t count, uint16_t* hi, uint16_t* lo);
 int mfma_rows(int np);  // rows of the zero-padded A operand
+void mfma_swizzle(const double* a, int rows, int np, double* out);  // row-major -> fragment order
 
 }  // namespace twmc
 

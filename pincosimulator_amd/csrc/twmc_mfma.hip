@@ -19,6 +19,9 @@
 // state is re-staged in LDS as the BF16 hi/lo B operand; A = C^T (hi/lo) is
 // read from L2 (it is shared by every workgroup).
 //
+// A = C^T is stored zero-padded to 4 TPW 32 rows and in fragment order
+// (mfma_swizzle), so each wave's A fragment load is one contiguous 1 KiB read.
+//
 // MFMA lane layout (gfx950, checked on the device): A lane l holds row l%32,
 // k = (l/32)*8 + 0..7; B lane l holds column l%32, same k; result register r
 // of lane l is row (r/4)*8 + (l/32)*4 + r%4, column l%32.
@@ -197,19 +200,23 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
       // A has 4 TPW 32 zero-padded rows, so every wave runs all TPW tiles
       // without a guard; A fragments are double-buffered one 16-wide k block
       // ahead of the MFMAs that use them (np / 16 is even)
-      const size_t abase = (size_t)(wave * TPW * 32 + col) * np + half * 8;
+      // A is stored in fragment order (one contiguous 1 KiB block per row tile
+      // and 16-wide k block, lane-major), so every fragment load is coalesced
+      const int nkb = np / 16;
+      const size_t abase = ((size_t)(wave * TPW) * nkb * 64 + lane) * 8;
+      auto afrag = [&](int t, int kblk) { return abase + ((size_t)t * nkb + kblk) * 512; };
       bf16x8 ah0[TPW], al0[TPW], ah1[TPW], al1[TPW];
 #pragma unroll
       for (int t = 0; t < TPW; ++t) {
-        ah0[t] = *reinterpret_cast<const bf16x8*>(&Ahi[abase + (size_t)t * 32 * np]);
-        al0[t] = *reinterpret_cast<const bf16x8*>(&Alo[abase + (size_t)t * 32 * np]);
+        ah0[t] = *reinterpret_cast<const bf16x8*>(&Ahi[afrag(t, 0)]);
+        al0[t] = *reinterpret_cast<const bf16x8*>(&Alo[afrag(t, 0)]);
       }
       for (int kb = 0; kb < np; kb += 32) {
         // k block kb: buffer 0; prefetch kb + 16 into buffer 1
 #pragma unroll
         for (int t = 0; t < TPW; ++t) {
-          ah1[t] = *reinterpret_cast<const bf16x8*>(&Ahi[abase + (size_t)t * 32 * np + kb + 16]);
-          al1[t] = *reinterpret_cast<const bf16x8*>(&Alo[abase + (size_t)t * 32 * np + kb + 16]);
+          ah1[t] = *reinterpret_cast<const bf16x8*>(&Ahi[afrag(t, kb / 16 + 1)]);
+          al1[t] = *reinterpret_cast<const bf16x8*>(&Alo[afrag(t, kb / 16 + 1)]);
         }
         {
           const int ko = kb + half * 8;
@@ -231,8 +238,8 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
         const int kn = kb + 32 < np ? kb + 32 : kb + 16;
 #pragma unroll
         for (int t = 0; t < TPW; ++t) {
-          ah0[t] = *reinterpret_cast<const bf16x8*>(&Ahi[abase + (size_t)t * 32 * np + kn]);
-          al0[t] = *reinterpret_cast<const bf16x8*>(&Alo[abase + (size_t)t * 32 * np + kn]);
+          ah0[t] = *reinterpret_cast<const bf16x8*>(&Ahi[afrag(t, kn / 16)]);
+          al0[t] = *reinterpret_cast<const bf16x8*>(&Alo[afrag(t, kn / 16)]);
         }
         {
           const int ko = kb + 16 + half * 8;
@@ -320,6 +327,17 @@ int mfma_padded(int n) { return (n + 31) / 32 * 32; }
 
 // rows of the padded A operand: 4 waves x TPW tiles x 32
 int mfma_rows(int np) { return 4 * ((np / 32 + 3) / 4) * 32; }
+
+// row-major rows x np -> fragment order: for row tile T, 16-wide k block b
+// and lane l, 8 consecutive elements A[32 T + l % 32][16 b + 8 (l / 32) + e]
+void mfma_swizzle(const double* a, int rows, int np, double* out) {
+  const int nkb = np / 16;
+  for (int T = 0; T < rows / 32; ++T)
+    for (int b = 0; b < nkb; ++b)
+      for (int l = 0; l < 64; ++l)
+        for (int e = 0; e < 8; ++e)
+          out[(((size_t)T * nkb + b) * 64 + l) * 8 + e] = a[(size_t)(T * 32 + l % 32) * np + b * 16 + (l / 32) * 8 + e];
+}
 
 size_t mfma_lds_bytes(int np) {
   return (size_t)2 * 64 * (np + 8) * 2 + (size_t)3 * np * 8 + 4 * 32 * 2 * 8;
