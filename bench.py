@@ -165,8 +165,9 @@ def warmstart_summary(n_points, n_traj):
         "speedup_median": float(np.median(ratio)) if ok else None,
         "speedup_geomean": float(np.exp(np.log(ratio).mean())) if ok else None,
         "speedup_min": float(ratio.min()) if ok else None,
-        "criterion": "steps until <n> and g2(0) stay within 3 sigma of the cold run's t>=25 mean "
-                     "(frames every 100 steps; a warm run counts only if its t in [15,20] mean agrees "
+        "criterion": "steps until <n> and g2(0) enter and hold (10 frames) the 3 sigma band around the cold "
+                     "run's t>=25 mean (frames every 100 steps; warm and cold runs use independent "
+                     "trajectories; a warm run counts only if its t in [15,20] mean agrees "
                      "with the cold value within 3 sigma; otherwise ratio 1; a warm run already steady at its "
                      "first frame counts as one frame, 100 steps)",
         "network": "MLP (F, Delta, U, J, L) -> 3x256 GELU -> mean field + 32 radial n(|k|) bins, FP32, "

@@ -52,7 +52,10 @@ def test_steady_targets_recover_known_ensemble():
 def test_time_to_steady_rule():
     x = np.array([5.0, 3.0, 1.2, 1.01, 0.99, 1.3, 1.0, 1.0])
     se = np.full_like(x, 0.05)
-    assert ws.time_to_steady(x, se, 1.0, 0.0) == 6  # frame 5 leaves the band
+    # inside the band: F F F T T F T T
+    assert ws.time_to_steady(x, se, 1.0, 0.0, hold=2) == 3
+    assert ws.time_to_steady(x, se, 1.0, 0.0, hold=3) == 6  # frame 5 breaks the hold from 3
+    assert ws.time_to_steady(x, se, 1.0, 0.0, hold=10) == 6  # holds to the end of the run
     assert ws.time_to_steady(np.ones(4), se[:4], 1.0, 0.0) == 0
     assert ws.time_to_steady(np.array([1.0, 2.0]), se[:2], 1.0, 0.0) is None
 
