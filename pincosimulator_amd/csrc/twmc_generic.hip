@@ -70,6 +70,10 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
 
   const int npairs = (n + 1) / 2;
   double* part = g.partials + (size_t)blockIdx.x * kQ * n;
+  constexpr bool kStencil = sizeof(T) == 4;
+  // x = floor(i / ny) as umulhi(i, ceil(2^32 / ny)): exact for i ny < 2^32
+  const uint32_t ymagic = g.ny > 1 ? (uint32_t)((0x100000000ull + (uint64_t)g.ny - 1) / (uint64_t)g.ny) : 0u;
+  const T jval = T(-0.5 * g.kscale);  // J
   const T dt = T(g.dt), dt4 = T(g.dt4), kscale = T(g.kscale);
 
   for (int b = blockIdx.x; b < g.n_traj; b += gridDim.x) {
@@ -104,9 +108,12 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
       }
       __syncthreads();
       cpx<T>* cur = bufA;
-      if (g.hopping == 0) cur = fft2d<-1>(bufA, bufB, g, twx, twy);
+      const bool cap = (s == 0 && g.capture);
+      // FP64 applies the reference's FFT form every step; FP32 uses its exact
+      // nearest-neighbour form and transforms only for a frame's k-space sums
+      if (g.hopping == 0 && (!kStencil || cap)) cur = fft2d<-1>(bufA, bufB, g, twx, twy);
 
-      if (s == 0 && g.capture) {
+      if (cap) {
         // ensemble sums of the state before this step (a stored frame)
         if (g.hopping == 0) {
           for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -159,7 +166,21 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
       }
       if (s >= g.n_steps) break;
 
-      if (g.hopping == 0) {
+      if (g.hopping == 0 && kStencil) {
+        if (cap) {
+          // the transform overwrote bufA: stage beta again for the neighbour sums
+#pragma unroll
+          for (int j = 0; j < PPT; ++j) {
+            int p = threadIdx.x + kBlock * j;
+            if (p < npairs) {
+              bufA[2 * p] = beta[2 * j];
+              if (2 * p + 1 < n) bufA[2 * p + 1] = beta[2 * j + 1];
+            }
+          }
+          __syncthreads();
+        }
+        cur = bufA;
+      } else if (g.hopping == 0) {
         // K product: K = i kscale (fx cos kx + fy cos ky) (TWMCBaseSolver.cpp:80-88, :240)
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
           int kx = i / g.ny, ky = i - kx * g.ny;
@@ -212,10 +233,27 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
               cpx<T> z = beta[2 * j + h];
               cpx<T> hh = cur[i];
               if (g.hopping == 0) {
-                if constexpr (sizeof(T) == 8) {
-                  hh = mk(hh.re / T(n), hh.im / T(n));
+                if constexpr (kStencil) {
+                  // h = -i J S, S the sum over the periodic nearest neighbours
+                  // (a dimension of length 1 has none): IDFT(K DFT beta)/N exactly
+                  const int x = g.ny > 1 ? (int)__umulhi((uint32_t)i, ymagic) : i;
+                  const int y = i - x * g.ny;
+                  T sr = T(0), si = T(0);
+                  if (g.ny > 1) {
+                    const cpx<T> a = bufA[x * g.ny + (y + 1 == g.ny ? 0 : y + 1)];
+                    const cpx<T> c = bufA[x * g.ny + (y == 0 ? g.ny - 1 : y - 1)];
+                    sr = a.re + c.re;
+                    si = a.im + c.im;
+                  }
+                  if (g.nx > 1) {
+                    const cpx<T> a = bufA[(x + 1 == g.nx ? 0 : x + 1) * g.ny + y];
+                    const cpx<T> c = bufA[(x == 0 ? g.nx - 1 : x - 1) * g.ny + y];
+                    sr += a.re + c.re;
+                    si += a.im + c.im;
+                  }
+                  hh = mk(jval * si, -jval * sr);
                 } else {
-                  hh = mk(hh.re * T(g.inv_n), hh.im * T(g.inv_n));
+                  hh = mk(hh.re / T(n), hh.im / T(n));
                 }
               }
               cpx<T> r = rsl[i], u = Up[i], f = fv[i];
