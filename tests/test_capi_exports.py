@@ -38,3 +38,18 @@ def test_abi_version_and_error_paths_without_gpu():
     assert L.twmc_create(None, 0, None) == -1
     assert L.twmc_last_error(None)
     assert L.twmc_batch_advance(None, 1) == -1
+
+
+def test_warm_start_and_sweep_entry_points_reject_bad_arguments_without_gpu():
+    import ctypes as C
+    L = _lib.load()
+    ctx = C.c_void_p()
+    bad = _lib.MlpConfig(n_in=0, n_hidden=8, n_hidden_layers=1, n_out=2, seed=0)
+    assert L.twmc_mlp_create(C.byref(bad), 0, C.byref(ctx)) == -1
+    assert b"shape" in L.twmc_mlp_last_error(None)
+    assert L.twmc_mlp_create(None, 0, C.byref(ctx)) == -1
+    assert L.twmc_mlp_forward(None, None, 0, None) == -1
+    assert L.twmc_mlp_train(None, None, None, None, 0, None, None) == -1
+    assert L.twmc_sweep_create(None, 0, 0.1, 1, 1, 0, C.byref(ctx)) == -1
+    assert L.twmc_sweep_run(None, 0.0, 1.0, 0, None, None, None, None, None) == -1
+    assert L.twmc_sweep_mode_bins(None, 0, None) == -1
