@@ -284,12 +284,19 @@ def main():
         f"over {st['step_launches']} launches, reduce avg "
         f"{st['reduce_kernel_ms'] / max(st['reduce_launches'], 1):.3f} ms, grid {st['grid']}")
 
-    traffic = None
+    traffic, counters = None, {}
     pmc = os.path.join(ROOT, "profiles", f"{info['kernel']}_64x64_16384traj_pmc.json" if args.workload == "square64"
                        else "mfma_lieb384_8192traj_pmc.json")
     if os.path.exists(pmc):
         try:
-            traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
+            prof = json.load(open(pmc))
+            traffic = prof.get("hbm_bytes_per_launch")
+            # the kernel's own limiter, measured with rocprofv3 --pmc on the same workload
+            for key in ("valu_insts_per_site_step", "valu_issue_utilisation", "mfma_util"):
+                if key in prof:
+                    counters[key] = prof[key]
+            if counters:
+                counters["source"] = os.path.relpath(pmc, ROOT)
         except Exception:
             traffic = None
 
@@ -308,7 +315,7 @@ def main():
             "bound": bound, "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": achieved / peak, "traffic": traffic,
             "kernel": kname, "avg_launch_ms": avg_ms,
-            "flops_per_launch": flops_per_launch, "flops_basis": basis,
+            "flops_per_launch": flops_per_launch, "flops_basis": basis, "pmc": counters or None,
         },
     }
     eng.close()
