@@ -77,8 +77,9 @@ struct SweepArgs {
   int n_steps, capture, frame, n_frames;
 };
 
+// clears the work-queue counters of the capture and the step launch
 __global__ void sweep_reset_kernel(unsigned int* counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+  if (threadIdx.x < 2 && blockIdx.x == 0) counter[threadIdx.x] = 0u;
 }
 
 template <bool CAPTURE_ONLY>
@@ -107,15 +108,17 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
     const int ny = P.ny;
     const int m = nx * ny;
     const int npairs = m / 2;  // ny is even for every supported size
-    for (int i = threadIdx.x; i < ny; i += blockDim.x) {
-      twy[i] = g.tw_pool[P.tw_y + i];
-      kcy[i] = g.kc_pool[P.kc_y + i];
-    }
-    if (P.fft_x)
-      for (int i = threadIdx.x; i < P.nx; i += blockDim.x) {
-        twx[i] = g.tw_pool[P.tw_x + i];
-        kcx[i] = g.kc_pool[P.kc_x + i];
+    if (CAPTURE_ONLY) {  // FFT tables, for the frame's k-space sums only
+      for (int i = threadIdx.x; i < ny; i += blockDim.x) {
+        twy[i] = g.tw_pool[P.tw_y + i];
+        kcy[i] = g.kc_pool[P.kc_y + i];
       }
+      if (P.fft_x)
+        for (int i = threadIdx.x; i < P.nx; i += blockDim.x) {
+          twx[i] = g.tw_pool[P.tw_x + i];
+          kcx[i] = g.kc_pool[P.kc_x + i];
+        }
+    }
     __syncthreads();
 
     // trajectory and in-trajectory pair index of each owned pair
@@ -157,7 +160,9 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
       }
       __syncthreads();
 
-      const bool cap = (s == 0 && g.capture);
+      // the capture launch (CAPTURE_ONLY) records the frame and stops; the
+      // step launch never captures and needs neither bufB nor the FFT tables
+      const bool cap = CAPTURE_ONLY;
       if (cap) {
         // per-trajectory site means of |beta|^2 and |beta|^4 (FP64), summed
         // in a fixed order so that repeated runs agree bit for bit; for
@@ -341,7 +346,11 @@ __global__ void sweep_init_kernel(const SweepPointDev* points, int n_points, con
   }
 }
 
-size_t sweep_lds_bytes() { return 64 + 2 * kSweepMaxSites * 8 + (kMaxNx + kMaxNy) * (8 + 4); }
+// dynamic LDS: the capture launch holds bufA, bufB and the FFT tables; the
+// step launch only bufA (so more workgroups fit on a CU)
+size_t sweep_lds_bytes(bool capture) {
+  return capture ? 64 + 2 * kSweepMaxSites * 8 + (kMaxNx + kMaxNy) * (8 + 4) : 64 + kSweepMaxSites * 8;
+}
 
 }  // namespace
 }  // namespace twmc
@@ -370,7 +379,7 @@ struct SweepCtx {
        *d_ksum = nullptr;
   size_t ksum_cap = 0;
   size_t m_cap = 0;
-  int n_items = 0, grid = 0;
+  int n_items = 0, grid = 0, grid_cap = 0;
   twmc_stats stats{};
 };
 
@@ -456,13 +465,24 @@ int build_items(SweepCtx* c) {
   return TWMC_OK;
 }
 
-// one segment: reset the queue, then the persistent step kernel
+// one segment: reset the queues, the frame capture (if any), then the steps;
+// each persistent launch drains its own queue counter
 hipError_t launch_segment(SweepCtx* c, const SweepArgs& a) {
   hipLaunchKernelGGL(sweep_reset_kernel, dim3(1), dim3(64), 0, c->stream, (unsigned int*)c->d_counter);
-  if (a.n_steps > 0)
-    hipLaunchKernelGGL(sweep_step_kernel<false>, dim3(c->grid), dim3(kSweepBlock), sweep_lds_bytes(), c->stream, a);
-  else
-    hipLaunchKernelGGL(sweep_step_kernel<true>, dim3(c->grid), dim3(kSweepBlock), sweep_lds_bytes(), c->stream, a);
+  if (a.capture) {
+    SweepArgs ca = a;
+    ca.n_steps = 0;
+    ca.counter = (unsigned int*)c->d_counter;
+    hipLaunchKernelGGL(sweep_step_kernel<true>, dim3(c->grid_cap), dim3(kSweepBlock), sweep_lds_bytes(true), c->stream,
+                       ca);
+  }
+  if (a.n_steps > 0) {
+    SweepArgs sa = a;
+    sa.capture = 0;
+    sa.counter = (unsigned int*)c->d_counter + 1;
+    hipLaunchKernelGGL(sweep_step_kernel<false>, dim3(c->grid), dim3(kSweepBlock), sweep_lds_bytes(false), c->stream,
+                       sa);
+  }
   return hipGetLastError();
 }
 
@@ -594,10 +614,11 @@ int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt,
   if (e == hipSuccess) e = hipMemcpy(c->d_tw, c->tw_pool.data(), c->tw_pool.size() * sizeof(cpx<float>), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_kc, c->kc_pool.data(), c->kc_pool.size() * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_binit, binit.data(), binit.size() * 8, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipFuncSetAttribute((const void*)sweep_step_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sweep_lds_bytes());
-  if (e == hipSuccess) e = hipFuncSetAttribute((const void*)sweep_step_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sweep_lds_bytes());
-  int per_cu = 0, cus = 0;
-  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sweep_step_kernel<false>, kSweepBlock, sweep_lds_bytes());
+  if (e == hipSuccess) e = hipFuncSetAttribute((const void*)sweep_step_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sweep_lds_bytes(false));
+  if (e == hipSuccess) e = hipFuncSetAttribute((const void*)sweep_step_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sweep_lds_bytes(true));
+  int per_cu = 0, per_cu_cap = 0, cus = 0;
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sweep_step_kernel<false>, kSweepBlock, sweep_lds_bytes(false));
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_cap, (const void*)sweep_step_kernel<true>, kSweepBlock, sweep_lds_bytes(true));
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) {
     std::string m = std::string("sweep setup: ") + hipGetErrorString(e);
@@ -605,6 +626,7 @@ int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt,
     return sfail(nullptr, TWMC_ERR_HIP, m);
   }
   c->grid = std::max(1, per_cu) * cus;
+  c->grid_cap = std::max(1, per_cu_cap) * cus;
   int rc = build_items(c);
   if (rc) {
     std::string m = c->err;
@@ -742,7 +764,7 @@ int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, d
     c->stats.site_steps += (uint64_t)c->dev_pts[p].n_traj * c->dev_pts[p].nx * c->dev_pts[p].ny * i_step;
   c->stats.grid = c->grid;
   c->stats.block = kSweepBlock;
-  c->stats.lds_bytes = (int)sweep_lds_bytes();
+  c->stats.lds_bytes = (int)sweep_lds_bytes(false);
   if (kernel_ms) *kernel_ms = ms;
   if (n_frames_out) *n_frames_out = frames;
   if (traj_m2) SCHK(c, hipMemcpy(traj_m2, c->d_m2, need * 8, hipMemcpyDeviceToHost));
