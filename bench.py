@@ -81,12 +81,20 @@ def cpu_baseline(threads, spec=None, target_s=12.0):
     steps = max(5, int(target_s * rate / n_traj))
     t = oracle.time_batch(p, n_traj, threads, steps, faithful=True)
     value = n_traj * n_sites * steps / t
-    kind = "sparse coupling product" if spec.get("lattice") == "matrix" else "FFT hopping"
+    kind = ("sparse coupling product" if spec.get("lattice") == "matrix"
+            else "FFT hopping with the oracle's own FFT, FFTW not being installed")
+    cpu = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), cpu)
+    except OSError:
+        pass
     return {
         "value": value, "unit": "trajectory·site-steps/s", "cores": threads, "kind": "port",
         "sample": f"{n_traj} trajectories x {steps} steps of the same {n_sites}-site system "
                   f"({t:.1f} s wall), FP64, reference per-step structure ({kind}) with std::mt19937 "
-                  f"noise, one trajectory per task on {threads} std::threads",
+                  f"noise, one trajectory per task on {threads} std::threads ({cpu}; {os.cpu_count()} logical CPUs "
+                  f"visible, threads = min(16, logical CPUs))",
     }
 
 
