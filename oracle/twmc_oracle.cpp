@@ -128,15 +128,16 @@ struct Plan1D {
     cd* in = a;
     cd* out = b;
     long Ns = 1;
-    cd v[64];
-    cd wr[64];
+    // butterfly scratch sized by the largest radix (a prime factor can be long)
+    int rmax = 1;
+    for (int R : radix) rmax = R > rmax ? R : rmax;
+    std::vector<cd> v(rmax), wr(rmax), o(rmax);
     for (int R : radix) {
       long nR = n / R;
       for (long j = 0; j < nR; ++j) {
         long k = j % Ns;
         for (int r = 0; r < R; ++r) v[r] = in[j + r * nR] * w(r * k * (n / (Ns * R)), sign);
         // length-R DFT of v with exp(sign 2 pi i r q / R)
-        cd o[64];
         if (R == 2) {
           o[0] = v[0] + v[1];
           o[1] = v[0] - v[1];

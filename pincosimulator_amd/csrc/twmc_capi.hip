@@ -110,21 +110,6 @@ int hip_fail(Ctx* c, hipError_t e, const char* what) {
     if (e_ != hipSuccess) return hip_fail(c, e_, #expr); \
   } while (0)
 
-bool make_plan(int len, RadixPlan& P) {
-  P.len = len;
-  P.npass = 0;
-  int m = len;
-  const int order[] = {8, 4, 2, 3, 5};
-  for (int r : order) {
-    while (m % r == 0) {
-      if (P.npass >= kMaxRadixPasses) return false;
-      P.radix[P.npass++] = r;
-      m /= r;
-    }
-  }
-  return m == 1;
-}
-
 void expand_matrix(const twmc_matrix& m, int n, std::vector<double>& base, std::vector<double>& amp,
                    int& noise) {
   base.assign(2 * (size_t)n, 0.0);
@@ -355,9 +340,9 @@ int twmc_create(const twmc_params* p, int device, void** ctx_out) {
   c->noisy_params = (c->noise[0] || c->noise[1] || c->noise[2]) ? 1 : 0;
 
   if (p->hopping == TWMC_HOP_FFT) {
-    if (!make_plan(p->nx, c->px) || !make_plan(p->ny, c->py)) {
+    if (!make_radix_plan(p->nx, c->px) || !make_radix_plan(p->ny, c->py)) {
       delete c;
-      return fail(nullptr, TWMC_ERR_UNSUPPORTED, "FFT lengths must factor into 2, 3, 5");
+      return fail(nullptr, TWMC_ERR_UNSUPPORTED, "FFT length needs more radix passes than supported");
     }
   } else {
     c->px.len = c->py.len = 0;

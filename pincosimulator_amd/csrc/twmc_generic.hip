@@ -1,6 +1,6 @@
-// twmc_generic.hip -- LDS-resident TWMC step kernel for any lattice whose
-// FFT lengths factor into 2, 3, 4, 5, 8 (and for matrix hopping), FP32 or
-// FP64.  One workgroup advances one trajectory at a time; a persistent grid
+// twmc_generic.hip -- LDS-resident TWMC step kernel for a lattice of any
+// size (mixed-radix FFT: radices 2, 3, 4, 5, 8 and a direct-sum pass for each
+// larger prime factor) and for matrix hopping, FP32 or FP64.  One workgroup advances one trajectory at a time; a persistent grid
 // walks the batch.  This kernel is the FP64 parity path and the fallback for
 // shapes the register-resident kernel does not cover.
 //

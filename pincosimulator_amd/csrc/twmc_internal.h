@@ -13,12 +13,39 @@ namespace twmc {
 constexpr int kQ = 5;
 constexpr int kMaxRadixPasses = 16;
 
-// Factorisation of one FFT length into radices 8, 4, 2, 3, 5 (in that order).
+// Factorisation of one FFT length into radices 8, 4, 2, 3, 5 (in that order),
+// then the remaining prime factors in ascending order (direct-sum passes).
 struct RadixPlan {
   int len;
   int npass;
   int radix[kMaxRadixPasses];
 };
+
+// Fills P for a length len >= 1; false only when the factors need more than
+// kMaxRadixPasses passes (no length below 2^16 does).
+inline bool make_radix_plan(int len, RadixPlan& P) {
+  P.len = len;
+  P.npass = 0;
+  if (len < 1) return false;
+  int m = len;
+  auto take = [&](int r) {
+    while (m % r == 0) {
+      if (P.npass >= kMaxRadixPasses) return false;
+      P.radix[P.npass++] = r;
+      m /= r;
+    }
+    return true;
+  };
+  for (int r : {8, 4, 2, 3, 5})
+    if (!take(r)) return false;
+  for (int r = 7; r * r <= m; r += 2)
+    if (!take(r)) return false;
+  if (m > 1) {
+    if (P.npass >= kMaxRadixPasses) return false;
+    P.radix[P.npass++] = m;
+  }
+  return true;
+}
 
 // Everything a step kernel needs; plain data passed by value.
 struct StepArgs {

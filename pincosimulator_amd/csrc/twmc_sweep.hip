@@ -395,19 +395,6 @@ int sfail(SweepCtx* c, int code, const std::string& m) {
     if (e_ != hipSuccess) return sfail(c, TWMC_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
 
-bool plan(int len, RadixPlan& P) {
-  P.len = len;
-  P.npass = 0;
-  int m = len;
-  for (int r : {8, 4, 2, 3, 5})
-    while (m % r == 0) {
-      if (P.npass >= kMaxRadixPasses) return false;
-      P.radix[P.npass++] = r;
-      m /= r;
-    }
-  return m == 1;
-}
-
 int add_tables(SweepCtx* c, int L, int& tw_off, int& kc_off) {
   tw_off = (int)c->tw_pool.size();
   kc_off = (int)c->kc_pool.size();
@@ -555,11 +542,10 @@ int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt,
     P.ny = ny;
     P.fft_x = nx > 1 ? 1 : 0;
     if (nx * ny > kSweepMaxSites || ny > kMaxNy || nx > kMaxNx || (ny % 2) != 0 || q.n_traj == 0 ||
-        q.n_traj > (uint64_t)INT32_MAX || !plan(ny, P.py) || !plan(nx, P.px)) {
+        q.n_traj > (uint64_t)INT32_MAX || !make_radix_plan(ny, P.py) || !make_radix_plan(nx, P.px)) {
       delete c;
       return sfail(nullptr, TWMC_ERR_UNSUPPORTED,
-                   "sweep point " + std::to_string(p) + ": up to 4096 sites, 64 rows, rows of even length <= 512, "
-                   "sizes with factors 2, 3, 5");
+                   "sweep point " + std::to_string(p) + ": up to 4096 sites, 64 rows, rows of even length <= 512");
     }
     auto ty = table_for(ny);
     auto tx = table_for(nx);

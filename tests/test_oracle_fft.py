@@ -4,8 +4,8 @@ import pytest
 
 import oracle
 
-SIZES_1D = [1, 2, 3, 4, 5, 6, 8, 12, 16, 32, 48, 64, 96, 128, 256, 384, 512]
-SIZES_2D = [(2, 2), (4, 4), (8, 8), (12, 10), (16, 15), (16, 16), (3, 5), (48, 48)]
+SIZES_1D = [1, 2, 3, 4, 5, 6, 7, 8, 11, 12, 16, 32, 48, 49, 64, 67, 96, 97, 127, 128, 256, 384, 512]
+SIZES_2D = [(2, 2), (4, 4), (8, 8), (12, 10), (16, 15), (16, 16), (3, 5), (48, 48), (5, 7), (13, 11)]
 
 
 def _rand(shape, seed):
