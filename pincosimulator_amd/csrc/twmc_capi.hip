@@ -110,6 +110,27 @@ int hip_fail(Ctx* c, hipError_t e, const char* what) {
     if (e_ != hipSuccess) return hip_fail(c, e_, #expr); \
   } while (0)
 
+// Grid of a kernel that runs one trajectory per single-wave workgroup in a
+// grid-stride loop and is bound by instruction issue.  With w waves per SIMD a
+// grid of w * SIMDs takes ceil(B / (w SIMDs)) rounds, each costing about w
+// issue periods, so take the w (up to what fits) that minimises rounds * w,
+// the larger w on ties (more latency hiding).  16384 trajectories on 1024
+// SIMDs: two waves per SIMD, 8 trajectories each, no tail round.
+int balanced_wave_grid(size_t B, int per_cu, int cus) {
+  const size_t simds = 4 * (size_t)cus;
+  const int wmax = std::max(1, per_cu / 4);
+  size_t best_cost = SIZE_MAX, best = simds;
+  for (int w = 1; w <= wmax; ++w) {
+    const size_t g = (size_t)w * simds;
+    const size_t cost = ((B + g - 1) / g) * (size_t)w;
+    if (cost <= best_cost) {
+      best_cost = cost;
+      best = g;
+    }
+  }
+  return (int)std::max<size_t>(1, std::min(best, B));
+}
+
 void expand_matrix(const twmc_matrix& m, int n, std::vector<double>& base, std::vector<double>& amp,
                    int& noise) {
   base.assign(2 * (size_t)n, 0.0);
@@ -566,7 +587,10 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
     HIPCHK(c, generic_occupancy(c->precision, n, c->block, c->lds, &per_cu));
   }
   if (per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");
-  c->grid = (int)std::min<size_t>(units, (size_t)per_cu * cus);
+  if (c->kernel == TWMC_KERNEL_STENCIL64 || c->kernel == TWMC_KERNEL_FUSED64)
+    c->grid = balanced_wave_grid(B, per_cu, cus);
+  else
+    c->grid = (int)std::min<size_t>(units, (size_t)per_cu * cus);
 
   size_t sets = c->noisy_params ? B : 1;
   HIPCHK(c, upload(c->d_ids, d->traj_ids, B * sizeof(uint64_t), s));

@@ -109,6 +109,23 @@ __device__ __forceinline__ void box_muller_f32(uint32_t a, uint32_t b, float& re
   im = r * __builtin_amdgcn_sinf(th);
 }
 
+// Box-Muller in the form the FP32 update kernels fold into their FMA chains:
+// rs = sigma r = sqrt(nl log2(u1)) with nl = -2 ln2 sigma^2 (so sigma costs no
+// instruction), and cos / sin of the angle.  u1 is mapped as in
+// box_muller_f32.  The angle is the top 23 bits of b as the float 1.f in
+// [1, 2) -- one v_alignbit_b32 -- since v_sin_f32 / v_cos_f32 take revolutions
+// and drop the integer part; it differs from b 2^-32 by less than 2^-23 turns.
+__device__ __forceinline__ void box_muller_f32_scaled(uint32_t a, uint32_t b, float nl, float& rs, float& c,
+                                                      float& s) {
+  const float two_m32 = 2.3283064365386963e-10f;
+  const float two_m33 = 1.1641532182693481e-10f;
+  const float u1 = fmaf(float(a), two_m32, two_m33);
+  rs = __builtin_amdgcn_sqrtf(nl * __builtin_amdgcn_logf(u1));
+  const float th = __builtin_bit_cast(float, __builtin_amdgcn_alignbit(0x7fu, b, 9u));
+  c = __builtin_amdgcn_cosf(th);
+  s = __builtin_amdgcn_sinf(th);
+}
+
 }  // namespace twmc
 
 #endif

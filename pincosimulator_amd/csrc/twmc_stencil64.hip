@@ -27,21 +27,31 @@ namespace {
 
 constexpr int kL = 64;
 
-__device__ __forceinline__ float lane_from(float v, int src) { return __shfl(v, src, 64); }
+// Whole-wave rotations by one lane (DPP wave_rol:1 / wave_ror:1): lane x
+// reads lane x + 1 or x - 1, wrapping at 64.  Used as the first operand of an
+// add, the compiler folds each into a v_add_f32_dpp, so the x-neighbours cost
+// no LDS traffic and no separate instruction.
+__device__ __forceinline__ float rot_up(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x134, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float rot_down(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x13c, 0xf, 0xf, true));
+}
 
 __global__ void __launch_bounds__(64, 2) stencil64_step_kernel(StepArgs g) {
   const int lane = threadIdx.x;  // row x
-  const int xm = (lane + 63) & 63, xp = (lane + 1) & 63;
 
   // uniform site parameters (the kernel is chosen only when they are uniform)
   const cpx<float> r0 = reinterpret_cast<const cpx<float>*>(g.rsl)[0];
   const cpx<float> u0 = reinterpret_cast<const cpx<float>*>(g.U)[0];
   const cpx<float> f0 = reinterpret_cast<const cpx<float>*>(g.F)[0];
   const float dtf = float(g.dt);
-  const float A = dtf * r0.re, Bc = dtf * r0.im;    // dt * rsl
   const float Ur = dtf * u0.re, Ui = dtf * u0.im;   // dt * U
+  // dt (rsl + i U (|b|^2 - 1)) = (A0 - Ui |b|^2) + i (B0 + Ur |b|^2)
+  const float A0 = fmaf(dtf, r0.re, Ui), B0 = fmaf(dtf, r0.im, -Ur);
   const float fr = -dtf * f0.im, fi = dtf * f0.re;  // dt * i F
   const float sig = float(g.dt4);
+  const float nl = -1.3862943611198906f * sig * sig;  // -2 ln2 sigma^2
   // dt h = -i J dt S: the FFT form's -2J (kscale) times the 1/2 of each cosine
   const float jdt = float(-0.5 * g.kscale * g.dt);
 
@@ -63,27 +73,40 @@ __global__ void __launch_bounds__(64, 2) stencil64_step_kernel(StepArgs g) {
       asm volatile("" : "+v"(pbase));
       const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
       const cpx<float> first = beta[0];  // old value of the y + 1 wrap of y = 63
-      cpx<float> prev = beta[kL - 1];     // old value of y - 1
+      // y-neighbour sum of the next site, formed from old values before the
+      // site between them is overwritten (so each new value can take its old
+      // value's register)
+      float syr = beta[kL - 1].re + beta[1].re, syi = beta[kL - 1].im + beta[1].im;
 #pragma unroll
       for (int p = 0; p < kL / 2; ++p) {
         const u32x4 w = philox4x32_10(make_counter(traj, TAG_WIENER, step, pbase + (uint32_t)p), g.key0, g.key1);
-        float xr[2], xi[2];
-        box_muller_f32(w.x, w.y, xr[0], xi[0]);
-        box_muller_f32(w.z, w.w, xr[1], xi[1]);
+        float rs[2], cs[2], sn[2];
+        box_muller_f32_scaled(w.x, w.y, nl, rs[0], cs[0], sn[0]);
+        box_muller_f32_scaled(w.z, w.w, nl, rs[1], cs[1], sn[1]);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int y = 2 * p + h;
           const cpx<float> v = beta[y];
-          const cpx<float> yn = (y == kL - 1) ? first : beta[y + 1];
           // every lane reads its neighbours' old beta[y] before any lane writes it
-          const float sr = lane_from(v.re, xm) + lane_from(v.re, xp) + prev.re + yn.re;
-          const float si = lane_from(v.im, xm) + lane_from(v.im, xp) + prev.im + yn.im;
-          prev = v;
-          const float q = v.re * v.re + v.im * v.im - 1.0f;
-          const float cr = A - Ui * q, ci = Bc + Ur * q;  // dt (rsl + i U (|b|^2 - 1))
-          const float nr = v.re + (cr * v.re - ci * v.im) + fmaf(jdt, si, fr) + sig * xr[h];
-          const float ni = v.im + (cr * v.im + ci * v.re) + fmaf(-jdt, sr, fi) + sig * xi[h];
-          beta[y] = mk(nr, ni);
+          const float sr = rot_down(v.re) + (rot_up(v.re) + syr);
+          const float si = rot_down(v.im) + (rot_up(v.im) + syi);
+          if (y < kL - 1) {
+            const cpx<float> y2 = (y + 2 == kL) ? first : beta[y + 2];
+            syr = v.re + y2.re;
+            syi = v.im + y2.im;
+          }
+          const float m = fmaf(v.im, v.im, v.re * v.re);
+          const float cr = fmaf(-Ui, m, A0), ci = fmaf(Ur, m, B0);
+          // increment = noise + dt iF + drift + hopping, all small, added to beta once
+          float dr = fmaf(rs[h], cs[h], fr);
+          dr = fmaf(cr, v.re, dr);
+          dr = fmaf(-ci, v.im, dr);
+          dr = fmaf(jdt, si, dr);
+          float di = fmaf(rs[h], sn[h], fi);
+          di = fmaf(cr, v.im, di);
+          di = fmaf(ci, v.re, di);
+          di = fmaf(-jdt, sr, di);
+          beta[y] = mk(v.re + dr, v.im + di);
         }
       }
     }
