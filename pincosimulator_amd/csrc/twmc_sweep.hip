@@ -147,6 +147,14 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
     // of the reference's IDFT(K DFT(beta)) / N (TWMCBaseSolver.cpp:80-88, :235-242)
     const float jre = -0.5f * P.kscale;  // J
     const bool hop_y = ny > 1, hop_x = P.fft_x && nx > 1;
+    // per-point step constants: dt (rsl + i U (|b|^2 - 1)) = (A0 - Ui |b|^2) + i (B0 + Ur |b|^2),
+    // dt iF = (fr, fi), dt h = -i jdt S, sigma^2 folded into the Box-Muller log multiplier
+    const float dt = g.dt;
+    const float Ur = dt * P.u_re, Ui = dt * P.u_im;
+    const float A0 = fmaf(dt, P.rsl_re, Ui), B0 = fmaf(dt, P.rsl_im, -Ur);
+    const float fr = -dt * P.f_im, fi = dt * P.f_re;
+    const float jdt = dt * jre;
+    const float nl = -1.3862943611198906f * P.dt4 * P.dt4;
 
     const int iters = CAPTURE_ONLY ? 1 : g.n_steps;
     for (int s = 0; s < iters; ++s) {
@@ -271,7 +279,6 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
       if (CAPTURE_ONLY) break;
 
       const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
-      const float dt = g.dt;
 #pragma unroll
       for (int j = 0; j < kPPT; ++j) {
         const int p = threadIdx.x + kSweepBlock * j;
@@ -280,9 +287,9 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
           const uint32_t pair = P.fft_x ? (uint32_t)p : (uint32_t)(p - row * (ny / 2));
           const uint64_t traj = P.traj_id0 + (uint64_t)(item.t0 + row);
           const u32x4 w = philox4x32_10(make_counter(traj, TAG_WIENER, step, pair), g.key0, g.key1);
-          float xr[2], xi[2];
-          box_muller_f32(w.x, w.y, xr[0], xi[0]);
-          box_muller_f32(w.z, w.w, xr[1], xi[1]);
+          float rs[2], cs[2], sn[2];
+          box_muller_f32_scaled(w.x, w.y, nl, rs[0], cs[0], sn[0]);
+          box_muller_f32_scaled(w.z, w.w, nl, rs[1], cs[1], sn[1]);
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             const int col = pcol[j] + h, rb = prow[j] * ny;
@@ -301,13 +308,18 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
               sr += a.re + b.re;
               si += a.im + b.im;
             }
-            const cpx<float> hh = mk(jre * si, -jre * sr);
-            const float q = z.re * z.re + z.im * z.im - 1.0f;
-            const cpx<float> c = mk(P.rsl_re + (-P.u_im) * q, P.rsl_im + P.u_re * q);
-            const cpx<float> cb = cmul(c, z);
-            const cpx<float> sdr = mk(cb.re + hh.re + (-P.f_im), cb.im + hh.im + P.f_re);
-            z = mk(z.re + sdr.re * dt + P.dt4 * xr[h], z.im + sdr.im * dt + P.dt4 * xi[h]);
-            beta[2 * j + h] = z;
+            const float mq = fmaf(z.im, z.im, z.re * z.re);
+            const float cr = fmaf(-Ui, mq, A0), ci = fmaf(Ur, mq, B0);
+            // increment = noise + dt iF + drift + hopping, all small, added to beta once
+            float dr = fmaf(rs[h], cs[h], fr);
+            dr = fmaf(cr, z.re, dr);
+            dr = fmaf(-ci, z.im, dr);
+            dr = fmaf(jdt, si, dr);
+            float di = fmaf(rs[h], sn[h], fi);
+            di = fmaf(cr, z.im, di);
+            di = fmaf(ci, z.re, di);
+            di = fmaf(-jdt, sr, di);
+            beta[2 * j + h] = mk(z.re + dr, z.im + di);
           }
         }
       }
