@@ -204,11 +204,11 @@ int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt,
                       uint64_t seed, int device, void** ctx_out);
 void twmc_sweep_destroy(void* ctx);
 const char* twmc_sweep_last_error(const void* ctx);
-/* trajectories stacked per work item (1D points; 2D points always 1) */
+/* trajectories stacked per step work item, per point (tb x sites <= 4096) */
 int twmc_sweep_set_batching(void* ctx, const int32_t* tb);
 int twmc_sweep_get_batching(void* ctx, int32_t* tb);
-/* times the stacking candidates of every 1D shape for `steps` steps and keeps
- * the fastest; tb_out (n_points, may be NULL) receives the choice */
+/* times the stacking candidates of every lattice shape for `steps` steps and
+ * keeps the fastest; tb_out (n_points, may be NULL) receives the choice */
 int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out);
 /* vacuum-start run from t_start to t_end with the stored initial frame;
  * traj_m2 / traj_m4 (total trajectories x n_frames, points in order) receive

@@ -4,11 +4,13 @@
 //
 // Work items are (point, block of trajectories).  Workgroups take items from a
 // global counter in decreasing order of cost (largest lattices first), so the
-// uneven items balance across the chip.  An item of a 2D point holds one
-// trajectory; an item of a 1D chain holds tb trajectories stacked as the rows
-// of a tb x N array, transformed along the rows only, so short chains still
-// fill a workgroup.  tb per lattice shape is chosen by twmc_sweep_autotune,
-// which times the candidates on the device.  The per-step arithmetic is the
+// uneven items balance across the chip.  A step item holds tb trajectories of
+// one point stacked as the rows of one array (a chain's trajectory is one
+// row, a 2D point's nx rows), so small lattices still fill a workgroup; the
+// frame capture takes 2D points one trajectory per item (its FFT runs along
+// both dimensions) and chains as stacked rows, transformed along the rows
+// only.  tb per lattice shape is chosen by twmc_sweep_autotune, which times
+// the candidates on the device.  The per-step arithmetic is the
 // LDS kernel's (same Stockham passes, K product, Philox increments, update),
 // so every trajectory of a sweep equals the same trajectory run alone.
 // The frame loop (counter reset + step kernel per frame) can be captured once
@@ -82,6 +84,101 @@ __global__ void sweep_reset_kernel(unsigned int* counter) {
   if (threadIdx.x < 2 && blockIdx.x == 0) counter[threadIdx.x] = 0u;
 }
 
+// One work item advanced g.n_steps Euler-Maruyama steps (the step launch).
+// The item is ntr trajectories of nxt rows each, stacked: 2D points have
+// nxt = nx lattice rows and x-hopping (XHOP), chains one row each and no
+// x-hopping; MULTI = false is the single-trajectory case (one trajectory id
+// for the whole item).  Thread t owns the site pairs p = t + 256 j, sites 2p
+// and 2p + 1 of one row (rows have even length), so the pair's two values are
+// each other's y-neighbours; the other y-neighbours, and the x-neighbours as
+// the 16-byte pairs of the rows above and below in the same trajectory, come
+// from the LDS copy of the state.  The hopping h = -i J S, S the sum over the
+// periodic nearest neighbours, is the exact real-space form of the reference's
+// IDFT(K DFT(beta)) / N (TWMCBaseSolver.cpp:80-88, :235-242); a length-2
+// dimension gives the one neighbour twice, as the FFT form does.
+template <bool XHOP, bool MULTI>
+__device__ __forceinline__ void step_item(const SweepArgs& g, const SweepPointDev& P, const SweepItemDev& item,
+                                          cpx<float>* bufA, cpx<float> (&beta)[2 * kPPT], int nxt, int ny,
+                                          int npairs) {
+  // neighbour offsets and Philox counters of the owned pairs, fixed for the item
+  int lft[kPPT], rgt[kPPT], up[kPPT], dn[kPPT];
+  uint32_t pid[kPPT];
+  uint64_t trj[kPPT];
+  const uint64_t traj0 = P.traj_id0 + (uint64_t)item.t0;
+#pragma unroll
+  for (int j = 0; j < kPPT; ++j) {
+    const int p = threadIdx.x + kSweepBlock * j;
+    const int row = (2 * p) / ny, col = 2 * p - row * ny;
+    const int tr = MULTI ? row / nxt : 0, r0 = tr * nxt, r = row - r0;
+    lft[j] = row * ny + (col == 0 ? ny - 1 : col - 1);
+    rgt[j] = row * ny + (col + 2 == ny ? 0 : col + 2);
+    up[j] = ((r0 + (r == 0 ? nxt - 1 : r - 1)) * ny + col) / 2;  // pair index
+    dn[j] = ((r0 + (r + 1 == nxt ? 0 : r + 1)) * ny + col) / 2;
+    pid[j] = (uint32_t)(p - tr * (nxt * ny / 2));
+    trj[j] = traj0 + (uint64_t)tr;
+  }
+  // step constants: dt (rsl + i U (|b|^2 - 1)) = (A0 - Ui |b|^2) + i (B0 + Ur |b|^2),
+  // dt iF = (fr, fi), dt h = -i jdt S, sigma^2 folded into the Box-Muller log multiplier
+  const float dt = g.dt;
+  const float Ur = dt * P.u_re, Ui = dt * P.u_im;
+  const float A0 = fmaf(dt, P.rsl_re, Ui), B0 = fmaf(dt, P.rsl_im, -Ur);
+  const float fr = -dt * P.f_im, fi = dt * P.f_re;
+  const float jdt = dt * (-0.5f * P.kscale);  // dt J
+  const float nl = -1.3862943611198906f * P.dt4 * P.dt4;
+  float4* pairs = reinterpret_cast<float4*>(bufA);
+
+  for (int s = 0; s < g.n_steps; ++s) {
+#pragma unroll
+    for (int j = 0; j < kPPT; ++j) {
+      const int p = threadIdx.x + kSweepBlock * j;
+      if (p < npairs) pairs[p] = make_float4(beta[2 * j].re, beta[2 * j].im, beta[2 * j + 1].re, beta[2 * j + 1].im);
+    }
+    __syncthreads();
+    const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
+#pragma unroll
+    for (int j = 0; j < kPPT; ++j) {
+      const int p = threadIdx.x + kSweepBlock * j;
+      if (p < npairs) {
+        const u32x4 w = philox4x32_10(make_counter(trj[j], TAG_WIENER, step, pid[j]), g.key0, g.key1);
+        float rs[2], cs[2], sn[2];
+        box_muller_f32_scaled(w.x, w.y, nl, rs[0], cs[0], sn[0]);
+        box_muller_f32_scaled(w.z, w.w, nl, rs[1], cs[1], sn[1]);
+        const cpx<float> z0 = beta[2 * j], z1 = beta[2 * j + 1];
+        const cpx<float> L = bufA[lft[j]], R = bufA[rgt[j]];
+        float sr[2], si[2];
+        sr[0] = z1.re + L.re;
+        si[0] = z1.im + L.im;
+        sr[1] = R.re + z0.re;
+        si[1] = R.im + z0.im;
+        if (XHOP) {
+          const float4 u = pairs[up[j]], d = pairs[dn[j]];
+          sr[0] += d.x + u.x;
+          si[0] += d.y + u.y;
+          sr[1] += d.z + u.z;
+          si[1] += d.w + u.w;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const cpx<float> z = h ? z1 : z0;
+          const float mq = fmaf(z.im, z.im, z.re * z.re);
+          const float cr = fmaf(-Ui, mq, A0), ci = fmaf(Ur, mq, B0);
+          // increment = noise + dt iF + drift + hopping, all small, added to beta once
+          float dr = fmaf(rs[h], cs[h], fr);
+          dr = fmaf(cr, z.re, dr);
+          dr = fmaf(-ci, z.im, dr);
+          dr = fmaf(jdt, si[h], dr);
+          float di = fmaf(rs[h], sn[h], fi);
+          di = fmaf(cr, z.im, di);
+          di = fmaf(ci, z.re, di);
+          di = fmaf(-jdt, sr[h], di);
+          beta[2 * j + h] = mk(z.re + dr, z.im + di);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 template <bool CAPTURE_ONLY>
 __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -103,8 +200,12 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
     if (it >= g.n_items) break;  // every workgroup reaches this once the queue is drained
     const SweepItemDev item = g.items[it];
     const SweepPointDev P = g.points[item.point];
-    const int rows_traj = P.fft_x ? 1 : min(P.tb, P.n_traj - item.t0);
-    const int nx = P.fft_x ? P.nx : rows_traj;  // rows of the item array
+    // trajectories in the item: the step launch stacks P.tb of every point,
+    // the capture launch takes 2D points one trajectory at a time
+    const int ntr = (CAPTURE_ONLY && P.fft_x) ? 1 : min(P.tb, P.n_traj - item.t0);
+    const int nxt = P.fft_x ? P.nx : 1;        // rows per trajectory
+    const int rows_traj = P.fft_x ? 1 : ntr;  // chains: one trajectory per row
+    const int nx = nxt * ntr;                  // rows of the item array
     const int ny = P.ny;
     const int m = nx * ny;
     const int npairs = m / 2;  // ny is even for every supported size
@@ -121,8 +222,8 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
     }
     __syncthreads();
 
-    // trajectory and in-trajectory pair index of each owned pair
-    cpx<float>* gbeta = g.beta + P.state_off + (size_t)item.t0 * (P.fft_x ? m : ny);
+    // the item's state: this thread's pairs p = t + 256 j in registers
+    cpx<float>* gbeta = g.beta + P.state_off + (size_t)item.t0 * nxt * ny;
     cpx<float> beta[2 * kPPT];
 #pragma unroll
     for (int j = 0; j < kPPT; ++j) {
@@ -133,31 +234,25 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
       }
     }
 
-    // row and column of each owned pair's even site (ny is even, so the odd
-    // site is the next column of the same row)
-    int prow[kPPT], pcol[kPPT];
+    if constexpr (!CAPTURE_ONLY) {
+      // the step launch: no capture, neither bufB nor the FFT tables
+      if (!P.fft_x)
+        step_item<false, true>(g, P, item, bufA, beta, 1, ny, npairs);
+      else if (ntr == 1)
+        step_item<true, false>(g, P, item, bufA, beta, nxt, ny, npairs);
+      else
+        step_item<true, true>(g, P, item, bufA, beta, nxt, ny, npairs);
 #pragma unroll
-    for (int j = 0; j < kPPT; ++j) {
-      const int p = threadIdx.x + kSweepBlock * j;
-      prow[j] = (2 * p) / ny;
-      pcol[j] = 2 * p - prow[j] * ny;
-    }
-    // hopping h = -i J S, S the sum over nearest neighbours of the lattice
-    // (periodic; a dimension of length 1 has none): the exact real-space form
-    // of the reference's IDFT(K DFT(beta)) / N (TWMCBaseSolver.cpp:80-88, :235-242)
-    const float jre = -0.5f * P.kscale;  // J
-    const bool hop_y = ny > 1, hop_x = P.fft_x && nx > 1;
-    // per-point step constants: dt (rsl + i U (|b|^2 - 1)) = (A0 - Ui |b|^2) + i (B0 + Ur |b|^2),
-    // dt iF = (fr, fi), dt h = -i jdt S, sigma^2 folded into the Box-Muller log multiplier
-    const float dt = g.dt;
-    const float Ur = dt * P.u_re, Ui = dt * P.u_im;
-    const float A0 = fmaf(dt, P.rsl_re, Ui), B0 = fmaf(dt, P.rsl_im, -Ur);
-    const float fr = -dt * P.f_im, fi = dt * P.f_re;
-    const float jdt = dt * jre;
-    const float nl = -1.3862943611198906f * P.dt4 * P.dt4;
-
-    const int iters = CAPTURE_ONLY ? 1 : g.n_steps;
-    for (int s = 0; s < iters; ++s) {
+      for (int j = 0; j < kPPT; ++j) {
+        const int p = threadIdx.x + kSweepBlock * j;
+        if (p < npairs) {
+          gbeta[2 * p] = beta[2 * j];
+          gbeta[2 * p + 1] = beta[2 * j + 1];
+        }
+      }
+    } else {
+      // the capture launch records the frame: per-trajectory means, mean
+      // field and k-space bins of the state
 #pragma unroll
       for (int j = 0; j < kPPT; ++j) {
         const int p = threadIdx.x + kSweepBlock * j;
@@ -167,11 +262,7 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
         }
       }
       __syncthreads();
-
-      // the capture launch (CAPTURE_ONLY) records the frame and stops; the
-      // step launch never captures and needs neither bufB nor the FFT tables
-      const bool cap = CAPTURE_ONLY;
-      if (cap) {
+      {
         // per-trajectory site means of |beta|^2 and |beta|^4 (FP64), summed
         // in a fixed order so that repeated runs agree bit for bit; for
         // stacked chains every row of the item is a trajectory
@@ -248,7 +339,7 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
         }
       }
 
-      if (cap) {
+      {
         // k-space populations: forward FFT, rows (length ny) then columns
         // (length nx) for 2D points; |DFT beta|^2 per radial bin, summed over
         // this item's trajectories
@@ -265,75 +356,6 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
         __syncthreads();
         if (threadIdx.x < kBins + 2)
           atomicAdd(&g.ksum[((size_t)P.index * g.n_frames + g.frame) * (kBins + 2) + threadIdx.x], s_bins[threadIdx.x]);
-        // the transform overwrote bufA: stage beta again
-#pragma unroll
-        for (int j = 0; j < kPPT; ++j) {
-          const int p = threadIdx.x + kSweepBlock * j;
-          if (p < npairs) {
-            bufA[2 * p] = beta[2 * j];
-            bufA[2 * p + 1] = beta[2 * j + 1];
-          }
-        }
-        __syncthreads();
-      }
-      if (CAPTURE_ONLY) break;
-
-      const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
-#pragma unroll
-      for (int j = 0; j < kPPT; ++j) {
-        const int p = threadIdx.x + kSweepBlock * j;
-        if (p < npairs) {
-          const int row = P.fft_x ? 0 : (2 * p) / ny;
-          const uint32_t pair = P.fft_x ? (uint32_t)p : (uint32_t)(p - row * (ny / 2));
-          const uint64_t traj = P.traj_id0 + (uint64_t)(item.t0 + row);
-          const u32x4 w = philox4x32_10(make_counter(traj, TAG_WIENER, step, pair), g.key0, g.key1);
-          float rs[2], cs[2], sn[2];
-          box_muller_f32_scaled(w.x, w.y, nl, rs[0], cs[0], sn[0]);
-          box_muller_f32_scaled(w.z, w.w, nl, rs[1], cs[1], sn[1]);
-#pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const int col = pcol[j] + h, rb = prow[j] * ny;
-            cpx<float> z = beta[2 * j + h];
-            float sr = 0.0f, si = 0.0f;
-            if (hop_y) {
-              const cpx<float> a = bufA[rb + (col + 1 == ny ? 0 : col + 1)];
-              const cpx<float> b = bufA[rb + (col == 0 ? ny - 1 : col - 1)];
-              sr = a.re + b.re;
-              si = a.im + b.im;
-            }
-            if (hop_x) {
-              const int r = prow[j];
-              const cpx<float> a = bufA[(r + 1 == nx ? 0 : r + 1) * ny + col];
-              const cpx<float> b = bufA[(r == 0 ? nx - 1 : r - 1) * ny + col];
-              sr += a.re + b.re;
-              si += a.im + b.im;
-            }
-            const float mq = fmaf(z.im, z.im, z.re * z.re);
-            const float cr = fmaf(-Ui, mq, A0), ci = fmaf(Ur, mq, B0);
-            // increment = noise + dt iF + drift + hopping, all small, added to beta once
-            float dr = fmaf(rs[h], cs[h], fr);
-            dr = fmaf(cr, z.re, dr);
-            dr = fmaf(-ci, z.im, dr);
-            dr = fmaf(jdt, si, dr);
-            float di = fmaf(rs[h], sn[h], fi);
-            di = fmaf(cr, z.im, di);
-            di = fmaf(ci, z.re, di);
-            di = fmaf(-jdt, sr, di);
-            beta[2 * j + h] = mk(z.re + dr, z.im + di);
-          }
-        }
-      }
-      __syncthreads();
-    }
-
-    if (!CAPTURE_ONLY) {
-#pragma unroll
-      for (int j = 0; j < kPPT; ++j) {
-        const int p = threadIdx.x + kSweepBlock * j;
-        if (p < npairs) {
-          gbeta[2 * p] = beta[2 * j];
-          gbeta[2 * p + 1] = beta[2 * j + 1];
-        }
       }
     }
   }
@@ -391,7 +413,8 @@ struct SweepCtx {
        *d_ksum = nullptr;
   size_t ksum_cap = 0;
   size_t m_cap = 0;
-  int n_items = 0, grid = 0, grid_cap = 0;
+  int n_items = 0, n_items_cap = 0, grid = 0, grid_cap = 0;
+  void* d_items_cap = nullptr;  // the capture launch's items: 2D points one trajectory each
   twmc_stats stats{};
 };
 
@@ -438,27 +461,34 @@ int add_bins(SweepCtx* c, int nx, int ny) {
   return off;
 }
 
-// lays out points, work items (largest first) and device buffers for the current tb
+// lays out points and the two work-item lists (largest first) for the
+// current tb: the step launch stacks tb trajectories of every point per item,
+// the capture launch takes 2D points one trajectory per item (chains as in the
+// step launch)
 int build_items(SweepCtx* c) {
-  std::vector<std::pair<double, SweepItemDev>> items;
-  for (size_t p = 0; p < c->pts.size(); ++p) {
-    const SweepPointDev& P = c->dev_pts[p];
-    const int step = P.fft_x ? 1 : c->tb[p];
-    for (int t0 = 0; t0 < P.n_traj; t0 += step) {
-      int rows = P.fft_x ? 1 : std::min(step, P.n_traj - t0);
-      double cost = double(rows) * P.nx * P.ny * std::log2(double(P.nx * P.ny) + 1.0);
-      items.push_back({cost, SweepItemDev{(int)p, t0}});
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool cap = pass == 1;
+    std::vector<std::pair<double, SweepItemDev>> items;
+    for (size_t p = 0; p < c->pts.size(); ++p) {
+      const SweepPointDev& P = c->dev_pts[p];
+      const int step = (cap && P.fft_x) ? 1 : c->tb[p];
+      for (int t0 = 0; t0 < P.n_traj; t0 += step) {
+        const int rows = std::min(step, P.n_traj - t0);
+        const double cost = double(rows) * P.nx * P.ny * std::log2(double(P.nx * P.ny) + 1.0);
+        items.push_back({cost, SweepItemDev{(int)p, t0}});
+      }
     }
+    std::stable_sort(items.begin(), items.end(), [](auto& a, auto& b) { return a.first > b.first; });
+    std::vector<SweepItemDev> flat;
+    for (auto& it : items) flat.push_back(it.second);
+    void*& d = cap ? c->d_items_cap : c->d_items;
+    (cap ? c->n_items_cap : c->n_items) = (int)flat.size();
+    if (d) (void)hipFree(d);
+    d = nullptr;
+    SCHK(c, hipMalloc(&d, flat.size() * sizeof(SweepItemDev)));
+    SCHK(c, hipMemcpy(d, flat.data(), flat.size() * sizeof(SweepItemDev), hipMemcpyHostToDevice));
   }
-  std::stable_sort(items.begin(), items.end(), [](auto& a, auto& b) { return a.first > b.first; });
-  std::vector<SweepItemDev> flat;
-  for (auto& it : items) flat.push_back(it.second);
   for (size_t p = 0; p < c->pts.size(); ++p) c->dev_pts[p].tb = c->tb[p];
-  c->n_items = (int)flat.size();
-  if (c->d_items) (void)hipFree(c->d_items);
-  c->d_items = nullptr;
-  SCHK(c, hipMalloc(&c->d_items, flat.size() * sizeof(SweepItemDev)));
-  SCHK(c, hipMemcpy(c->d_items, flat.data(), flat.size() * sizeof(SweepItemDev), hipMemcpyHostToDevice));
   SCHK(c, hipMemcpy(c->d_points, c->dev_pts.data(), c->dev_pts.size() * sizeof(SweepPointDev),
                     hipMemcpyHostToDevice));
   return TWMC_OK;
@@ -472,6 +502,8 @@ hipError_t launch_segment(SweepCtx* c, const SweepArgs& a) {
     SweepArgs ca = a;
     ca.n_steps = 0;
     ca.counter = (unsigned int*)c->d_counter;
+    ca.items = (const SweepItemDev*)c->d_items_cap;
+    ca.n_items = c->n_items_cap;
     hipLaunchKernelGGL(sweep_step_kernel<true>, dim3(c->grid_cap), dim3(kSweepBlock), sweep_lds_bytes(true), c->stream,
                        ca);
   }
@@ -588,7 +620,7 @@ int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt,
     m_off += q.n_traj;  // times n_frames at run time
     c->dev_pts.push_back(P);
     // default stacking: fill a workgroup with about 2048 sites
-    c->tb.push_back(P.fft_x ? 1 : std::max(1, std::min({(int)q.n_traj, 2048 / ny, kMaxRows})));
+    c->tb.push_back(std::max(1, std::min({(int)q.n_traj, 2048 / (nx * ny), kMaxRows})));
   }
   c->total_traj = m_off;
   c->total_sites = state_off;
@@ -640,7 +672,7 @@ void twmc_sweep_destroy(void* ctx) {
   SweepCtx* c = static_cast<SweepCtx*>(ctx);
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* bufs[] = {c->d_points, c->d_items, c->d_counter, c->d_tw, c->d_kc, c->d_beta, c->d_m2, c->d_m4, c->d_binit,
+  void* bufs[] = {c->d_points, c->d_items, c->d_items_cap, c->d_counter, c->d_tw, c->d_kc, c->d_beta, c->d_m2, c->d_m4, c->d_binit,
                   c->d_bins, c->d_ksum};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -653,8 +685,8 @@ int twmc_sweep_set_batching(void* ctx, const int32_t* tb) {
   SweepCtx* c = static_cast<SweepCtx*>(ctx);
   for (size_t p = 0; p < c->pts.size(); ++p) {
     const SweepPointDev& P = c->dev_pts[p];
-    int v = P.fft_x ? 1 : tb[p];
-    if (v < 1 || v > kMaxRows || (int64_t)v * P.ny > kSweepMaxSites)
+    const int v = tb[p];
+    if (v < 1 || v > kMaxRows || (int64_t)v * P.nx * P.ny > kSweepMaxSites)
       return sfail(c, TWMC_ERR_INVALID, "stacking exceeds 4096 sites or 256 trajectories");
     c->tb[p] = std::min(v, P.n_traj);
   }
@@ -780,20 +812,23 @@ int twmc_sweep_mode_bins(void* ctx, uint64_t point, uint8_t* bins) {
   return TWMC_OK;
 }
 
-// Times each candidate stacking per 1D shape on a short run (steps steps of
-// every point of that shape) and keeps the fastest per site-step.
+// Times each candidate stacking per lattice shape on a short run (steps steps
+// of every point of that shape) and keeps the fastest per site-step.
 int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out) {
   if (!ctx) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
   SweepCtx* c = static_cast<SweepCtx*>(ctx);
   if (steps < 1) steps = 20;
-  std::map<int, std::vector<size_t>> by_len;  // 1D chain length -> points
-  for (size_t p = 0; p < c->pts.size(); ++p)
-    if (!c->dev_pts[p].fft_x) by_len[c->dev_pts[p].ny].push_back(p);
+  std::map<std::pair<int, int>, std::vector<size_t>> by_shape;  // (rows, row length) -> points
+  for (size_t p = 0; p < c->pts.size(); ++p) by_shape[{c->dev_pts[p].nx, c->dev_pts[p].ny}].push_back(p);
   std::vector<int32_t> best = c->tb;
-  for (auto& kv : by_len) {
-    const int ny = kv.first;
+  for (auto& kv : by_shape) {
+    const int sites = kv.first.first * kv.first.second;
+    if (2 * sites > kSweepMaxSites) {  // one trajectory per item is the only choice
+      for (size_t p : kv.second) best[p] = 1;
+      continue;
+    }
     double best_rate = -1.0;
-    for (int cand = 1; cand * ny <= kSweepMaxSites && cand <= kMaxRows; cand *= 2) {
+    for (int cand = 1; cand * sites <= kSweepMaxSites && cand <= kMaxRows; cand *= 2) {
       if (cand > c->dev_pts[kv.second[0]].n_traj && cand > 1) break;
       std::vector<int32_t> tb = c->tb;
       for (size_t p : kv.second) tb[p] = cand;

@@ -87,12 +87,29 @@ def test_autotune_changes_nothing_but_speed():
     tb = sw.autotune(steps=10)
     print("autotuned stacking per point:", tb)
     for k, p in enumerate(points()):
-        if p["nx"] > 1 and p["ny"] > 1:
-            assert tb[k] == 1
-        else:
-            n = max(p["nx"], p["ny"])
-            assert 1 <= tb[k] <= max(1, 4096 // n)
+        assert 1 <= tb[k] <= max(1, 4096 // (p["nx"] * p["ny"]))
     got = sw.run(0.2, graph=True)
     for k in range(len(points())):
         np.testing.assert_array_equal(got["m2"][k], ref["m2"][k])
+    sw.close()
+
+
+def test_stacked_2d_items_equal_unstacked():
+    # 2D points stacked two trajectories per step item: 3 and 5 trajectories
+    # give a last item of one trajectory (the single-trajectory path) beside
+    # stacked ones; every trajectory must equal the unstacked run bit for bit
+    pts = [dict(nx=16, ny=16, n_traj=3, traj_id0=2000, J=1.0, gamma=1.0, omega=1.0, U=0.05, F=2.0),
+           dict(nx=12, ny=10, n_traj=5, traj_id0=4000, J=1.0, gamma=1.0, omega=2.0, U=0.05, F=3.0),
+           dict(nx=2, ny=8, n_traj=5, traj_id0=5000, J=0.7, gamma=1.0, omega=0.3, U=0.05, F=1.0)]
+    sw = Sweep(pts, DT, FS, SEED)
+    sw.set_batching([1, 1, 1])
+    ref = sw.run(0.2, graph=False)
+    sw.set_batching([2, 2, 4])
+    assert sw.batching() == [2, 2, 4]
+    got = sw.run(0.2, graph=True)
+    for k in range(len(pts)):
+        np.testing.assert_array_equal(got["m2"][k], ref["m2"][k])
+        np.testing.assert_array_equal(got["m4"][k], ref["m4"][k])
+        # k-space bins are FP64 atomic sums over items: equal up to summation order
+        np.testing.assert_allclose(got["ksum"][k], ref["ksum"][k], rtol=1e-12, atol=1e-12)
     sw.close()
