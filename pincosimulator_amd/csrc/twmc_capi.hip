@@ -259,8 +259,10 @@ hipError_t timed(Ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>& evs, hi
   return r;
 }
 
-// stencil64 segment: the capture (if any) is a launch of the 64x64 FFT
-// kernel's capture-only variant plus the reduction, then the steps
+// stencil64 segment: the capture (if any) is the register-FFT capture kernel
+// (per-trajectory means, dump, k-space sums per wave), the site-major kernel
+// (per-site real-space sums per trajectory chunk) and the reduction of both
+// partial sets, then the steps
 int advance_stencil(Ctx* c, const StepArgs& a, hipStream_t s) {
   const int n = c->n;
   if (a.capture) {
@@ -268,10 +270,19 @@ int advance_stencil(Ctx* c, const StepArgs& a, hipStream_t s) {
     ca.n_steps = 0;
     hipEvent_t r0 = nullptr;
     HIPCHK(c, timed(c, c->ev_red, s, true, r0));
-    HIPCHK(c, launch_fused64(ca, c->cap_grid, s));
+    HIPCHK(c, launch_capture64(ca, c->cap_grid, s));
+    SiteSumArgs ss{};
+    ss.beta = a.beta;
+    ss.partials = static_cast<double*>(c->d_partials.p);
+    ss.n = n;
+    ss.n_traj = (int)c->n_traj;
+    const int chunks = site_sum_chunks(c->n_traj, kSiteSumChunks, &ss.chunk);
+    HIPCHK(c, launch_site_sums(ss, c->precision, s));
     ReduceArgs r{};
     r.n = n;
-    r.slots = c->cap_grid;
+    r.slots = chunks;
+    r.slots_k = c->cap_grid;
+    r.keep = 1;
     r.partials_in = static_cast<const double*>(c->d_partials.p);
     r.partials = static_cast<double*>(c->d_partials.p);
     r.obs = static_cast<double*>(c->d_obs.p) + (size_t)a.frame * kQ * n;
@@ -572,7 +583,7 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
     c->block = 64;
     c->lds = 0;
     int cap_per_cu = 0;
-    HIPCHK(c, fused64_occupancy(&cap_per_cu));
+    HIPCHK(c, capture64_occupancy(&cap_per_cu));
     if (cap_per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "capture kernel does not fit on a CU");
     c->cap_grid = (int)std::min<size_t>(B, (size_t)cap_per_cu * cus);
     HIPCHK(c, stencil64_occupancy(&per_cu));

@@ -230,7 +230,77 @@ __global__ void __launch_bounds__(64, 1) fused64_step_kernel(StepArgs g) {
   }
 }
 
+// Frame capture of the 64 x 64 stencil path.  Per trajectory: the site means
+// of |beta|^2 and |beta|^4 and, if asked, the frame dump; and |DFT beta|^2
+// summed over this wave's trajectories in FP64 registers (after the two
+// register FFTs lane holds ky = rev8(lane), slot q holds kx = rev8(q)),
+// written once per wave, in natural k order, as the k-space row of partial
+// slot blockIdx.x.  The per-site real-space sums come from the site-major
+// kernel beside it, so no partial row is read back here: every sum has a
+// fixed trajectory order and repeated runs agree bit for bit.
+__global__ void __launch_bounds__(64, 1) capture64_kernel(StepArgs g) {
+  __shared__ cpx<float> T[kL * kPad];
+  const int lane = threadIdx.x;
+  double acc[kL];
+#pragma unroll
+  for (int q = 0; q < kL; ++q) acc[q] = 0.0;
+
+  for (int b = blockIdx.x; b < g.n_traj; b += gridDim.x) {
+    const cpx<float>* gb = reinterpret_cast<const cpx<float>*>(g.beta) + (size_t)b * 4096;
+    // coalesced load through LDS: row r, column lane
+#pragma unroll
+    for (int r = 0; r < kL; ++r) T[r * kPad + lane] = gb[r * kL + lane];
+    __syncthreads();
+    if (g.dump) {
+      cpx<float>* out = reinterpret_cast<cpx<float>*>(g.traj_out) + ((size_t)b * g.n_frames + g.frame) * 4096;
+#pragma unroll
+      for (int r = 0; r < kL; ++r) out[r * kL + lane] = T[r * kPad + lane];
+    }
+    cpx<float> z[kL];  // lane x holds row x
+#pragma unroll
+    for (int y = 0; y < kL; ++y) z[y] = T[lane * kPad + y];
+    __syncthreads();
+
+    double m2 = 0.0, m4 = 0.0;
+#pragma unroll
+    for (int y = 0; y < kL; ++y) {
+      const double nn = double(z[y].re) * double(z[y].re) + double(z[y].im) * double(z[y].im);
+      m2 += nn;
+      m4 += nn * nn;
+    }
+    m2 = wave_sum(m2);
+    m4 = wave_sum(m4);
+    if (lane == 0) {
+      g.traj_m2[(size_t)b * g.n_frames + g.frame] = m2 / 4096.0;
+      g.traj_m4[(size_t)b * g.n_frames + g.frame] = m4 / 4096.0;
+    }
+
+    fft64<-1>(z);  // along y: slot q holds ky = rev8(q)
+#pragma unroll
+    for (int q = 0; q < kL; ++q) T[q * kPad + lane] = z[q];
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < kL; ++x) z[x] = T[lane * kPad + x];
+    __syncthreads();
+    fft64<-1>(z);  // along x: slot q holds kx = rev8(q), ky = rev8(lane)
+#pragma unroll
+    for (int q = 0; q < kL; ++q) acc[q] += double(z[q].re) * double(z[q].re) + double(z[q].im) * double(z[q].im);
+  }
+  double* part = g.partials + (size_t)blockIdx.x * kQ * 4096 + 2 * 4096;
+#pragma unroll
+  for (int q = 0; q < kL; ++q) part[rev8(q) * kL + rev8(lane)] = acc[q];
+}
+
 }  // namespace
+
+hipError_t launch_capture64(const StepArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(capture64_kernel, dim3(grid), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t capture64_occupancy(int* blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)capture64_kernel, 64, 0);
+}
 
 bool fused64_supported(int nx, int ny, int hopping, int precision, int param_stride) {
   return nx == 64 && ny == 64 && hopping == 0 && precision == 32 && param_stride == 0;

@@ -9,6 +9,8 @@
 //   K product, backward DFT, /N, drift + Philox increment, update in registers.
 // Matrix hopping (src/TWMC/Solvers/TWMCLiebSolver.cpp:259-267):
 //   h_j = -i sum_i beta_i C_ij over the stored nonzeros of column j.
+#include <algorithm>
+
 #include "twmc_cplx.h"
 #include "twmc_internal.h"
 #include "twmc_rng.h"
@@ -392,16 +394,45 @@ __global__ void __launch_bounds__(256) gauss_init_kernel(GaussArgs g) {
   }
 }
 
+// Per-site ensemble sums of one frame in site-major order: thread i of chunk
+// c sums |beta_i|^2, |beta_i|^4, Re beta_i and Im beta_i over the trajectories
+// of chunk c (a fixed range in a fixed order, so repeated runs agree bit for
+// bit) and writes rows 0, 1, 3 and 4 of partial slot c.  Consecutive threads
+// read consecutive sites of one trajectory: the state is read once, coalesced.
+template <typename T>
+__global__ void __launch_bounds__(256) site_sums_kernel(SiteSumArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int c = blockIdx.y;
+  const int b0 = c * a.chunk, b1 = min(a.n_traj, b0 + a.chunk);
+  const cpx<T>* beta = reinterpret_cast<const cpx<T>*>(a.beta);
+  double s2 = 0.0, s4 = 0.0, sr = 0.0, si = 0.0;
+  for (int b = b0; b < b1; ++b) {
+    const cpx<T> v = beta[(size_t)b * a.n + i];
+    const double re = double(v.re), im = double(v.im), nn = re * re + im * im;
+    s2 += nn;
+    s4 += nn * nn;
+    sr += re;
+    si += im;
+  }
+  double* part = a.partials + (size_t)c * kQ * a.n;
+  part[i] = s2;
+  part[a.n + i] = s4;
+  part[3 * a.n + i] = sr;
+  part[4 * a.n + i] = si;
+}
+
 __global__ void reduce_kernel(ReduceArgs a) {
   const int total = kQ * a.n;
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+    const int q = idx / a.n, j = idx - q * a.n;
+    const int ns = (q == 2 && a.slots_k > 0) ? a.slots_k : a.slots;
     double s = 0.0;
-    for (int sl = 0; sl < a.slots; ++sl) {
+    for (int sl = 0; sl < ns; ++sl) {
       size_t o = (size_t)sl * total + idx;
       s += a.partials_in[o];
-      a.partials[o] = 0.0;
+      if (!a.keep) a.partials[o] = 0.0;
     }
-    int q = idx / a.n, j = idx - q * a.n;
     const int* perm = (q == 2) ? a.perm_k : a.perm_site;
     int dst = perm ? perm[j] : j;
     a.obs[q * a.n + dst] = s;
@@ -518,6 +549,22 @@ hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {
   int block = 256;
   int grid = (total + block - 1) / block;
   hipLaunchKernelGGL(reduce_kernel, dim3(grid), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+
+int site_sum_chunks(uint64_t n_traj, int max_chunks, int* chunk) {
+  const int want = (int)std::max<uint64_t>(1, std::min<uint64_t>(n_traj, (uint64_t)max_chunks));
+  *chunk = (int)((n_traj + want - 1) / want);
+  return (int)((n_traj + *chunk - 1) / *chunk);
+}
+
+hipError_t launch_site_sums(const SiteSumArgs& a, int precision, hipStream_t s) {
+  const int chunks = (a.n_traj + a.chunk - 1) / a.chunk;
+  const dim3 grid((a.n + 255) / 256, chunks);
+  if (precision == 64)
+    hipLaunchKernelGGL(site_sums_kernel<double>, grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(site_sums_kernel<float>, grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -140,11 +140,27 @@ struct GaussArgs {
 struct ReduceArgs {
   int n, slots;
   const double* partials_in;  // [slots][kQ][n]
-  double* partials;           // same buffer, cleared after reading
+  double* partials;           // same buffer, cleared after reading unless keep
   double* obs;                // [kQ][n] for one frame
   const int* perm_site;       // native index -> natural index, or null
   const int* perm_k;
+  int slots_k;  // slots of the k-space row (q = 2) when > 0, else slots
+  int keep;     // 1: the producers overwrite their rows, no clearing
 };
+
+// per-site real-space sums in site-major order (rows 0, 1, 3, 4 of the slots);
+// at most this many trajectory chunks (partial slots) per frame
+constexpr int kSiteSumChunks = 128;
+struct SiteSumArgs {
+  const void* beta;  // [n_traj][n] complex, precision of the batch
+  double* partials;  // [chunks][kQ][n]
+  int n, n_traj, chunk;
+};
+hipError_t launch_site_sums(const SiteSumArgs& a, int precision, hipStream_t s);
+// chunks of at most max_chunks for n_traj trajectories; *chunk = trajectories per chunk
+int site_sum_chunks(uint64_t n_traj, int max_chunks, int* chunk);
+hipError_t launch_capture64(const StepArgs& a, int grid, hipStream_t s);
+hipError_t capture64_occupancy(int* blocks_per_cu);
 
 hipError_t launch_init(const InitArgs& a, int precision, hipStream_t s);
 hipError_t launch_gauss_init(const GaussArgs& a, int precision, int grid, hipStream_t s);

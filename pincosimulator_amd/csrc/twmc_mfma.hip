@@ -68,15 +68,22 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int col = lane & 31, half = lane >> 5;
   const int tiles = np / 32;  // wave w owns row tiles w*TPW .. w*TPW+TPW-1 that are < tiles
+  // site tables pre-scaled by dt for the epilogue's FMA chain:
+  //   dt (rsl + i U (|b|^2 - 1)) = (A - dt U.im |b|^2) + i (B + dt U.re |b|^2),
+  //   Prsl = (A, B) = (dt (rsl.re + U.im), dt (rsl.im - U.re)), PU = dt U, PF = dt i F
+  const float dt = float(g.dt), sig = float(g.dt4);
+  const float nl = -1.3862943611198906f * sig * sig;  // -2 ln2 sigma^2 (Box-Muller)
   for (int i = tid; i < np; i += blockDim.x) {
     const bool in = i < n;
-    Prsl[i] = in ? reinterpret_cast<const cpx<float>*>(g.rsl)[i] : mk(0.f, 0.f);
-    PU[i] = in ? reinterpret_cast<const cpx<float>*>(g.U)[i] : mk(0.f, 0.f);
-    PF[i] = in ? reinterpret_cast<const cpx<float>*>(g.F)[i] : mk(0.f, 0.f);
+    const cpx<float> r = in ? reinterpret_cast<const cpx<float>*>(g.rsl)[i] : mk(0.f, 0.f);
+    const cpx<float> u = in ? reinterpret_cast<const cpx<float>*>(g.U)[i] : mk(0.f, 0.f);
+    const cpx<float> f = in ? reinterpret_cast<const cpx<float>*>(g.F)[i] : mk(0.f, 0.f);
+    Prsl[i] = mk(dt * (r.re + u.im), dt * (r.im - u.re));
+    PU[i] = mk(dt * u.re, dt * u.im);
+    PF[i] = mk(-dt * f.im, dt * f.re);
   }
   __syncthreads();
 
-  const float dt = float(g.dt), sig = float(g.dt4);
   const uint16_t* Ahi = reinterpret_cast<const uint16_t*>(g.a_hi);
   const uint16_t* Alo = reinterpret_cast<const uint16_t*>(g.a_lo);
   double* part = g.partials + (size_t)blockIdx.x * kQ * n;
@@ -185,7 +192,8 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
         const float fdt = float(drive_interval(g.ft_times, g.ft_n, g.t0 + double(s) * g.dt, ft_id));
         const cpx<float>* fv = reinterpret_cast<const cpx<float>*>(g.ft_vals) + (size_t)ft_id * n;
         const cpx<float>* fs = reinterpret_cast<const cpx<float>*>(g.ft_slopes) + (size_t)ft_id * n;
-        for (int i = tid; i < n; i += blockDim.x) PF[i] = mk(fv[i].re + fdt * fs[i].re, fv[i].im + fdt * fs[i].im);
+        for (int i = tid; i < n; i += blockDim.x)
+          PF[i] = mk(-dt * fmaf(fdt, fs[i].im, fv[i].im), dt * fmaf(fdt, fs[i].re, fv[i].re));
       }
 
       f32x16 are[TPW], aim[TPW];
@@ -276,22 +284,30 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
             const int i0 = (wave * TPW + t) * 32 + q * 8 + half * 4 + 2 * pr;  // even site
             const u32x4 w = philox4x32_10(make_counter(traj, TAG_WIENER, step, pbase + (uint32_t)(t * 16 + q * 4 + pr)),
                                           g.key0, g.key1);
-            float xr[2], xi[2];
-            box_muller_f32(w.x, w.y, xr[0], xi[0]);
-            box_muller_f32(w.z, w.w, xr[1], xi[1]);
+            float rn[2], cs[2], sn[2];
+            box_muller_f32_scaled(w.x, w.y, nl, rn[0], cs[0], sn[0]);
+            box_muller_f32_scaled(w.z, w.w, nl, rn[1], cs[1], sn[1]);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
               const int r = 4 * q + 2 * pr + h;
               const int i = i0 + h;
               if (i < n) {
                 const float br = sre[t][r], bi = sim[t][r];
-                const cpx<float> rs = Prsl[i], u = PU[i], f = PF[i];
-                const float qq = br * br + bi * bi - 1.0f;
-                const float cr = rs.re - u.im * qq, ci = rs.im + u.re * qq;
-                const float dr = cr * br - ci * bi + aim[t][r] - f.im;  // h_re = Im Y, i F = (-F.im, F.re)
-                const float di = cr * bi + ci * br - are[t][r] + f.re;  // h_im = -Re Y
-                sre[t][r] = br + dt * dr + sig * xr[h];
-                sim[t][r] = bi + dt * di + sig * xi[h];
+                const cpx<float> ab = Prsl[i], u = PU[i], f = PF[i];
+                const float mq = fmaf(bi, bi, br * br);
+                const float cr = fmaf(-u.im, mq, ab.re), ci = fmaf(u.re, mq, ab.im);
+                // increment = noise + dt iF + dt h + drift, h = -i Y (h_re = Im Y, h_im = -Re Y),
+                // all small, added to beta once
+                float dr = fmaf(rn[h], cs[h], f.re);
+                dr = fmaf(dt, aim[t][r], dr);
+                dr = fmaf(cr, br, dr);
+                dr = fmaf(-ci, bi, dr);
+                float di = fmaf(rn[h], sn[h], f.im);
+                di = fmaf(-dt, are[t][r], di);
+                di = fmaf(cr, bi, di);
+                di = fmaf(ci, br, di);
+                sre[t][r] = br + dr;
+                sim[t][r] = bi + di;
               }
             }
           }
