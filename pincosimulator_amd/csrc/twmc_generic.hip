@@ -428,10 +428,24 @@ __global__ void reduce_kernel(ReduceArgs a) {
     const int q = idx / a.n, j = idx - q * a.n;
     const int ns = (q == 2 && a.slots_k > 0) ? a.slots_k : a.slots;
     double s = 0.0;
-    for (int sl = 0; sl < ns; ++sl) {
-      size_t o = (size_t)sl * total + idx;
-      s += a.partials_in[o];
-      if (!a.keep) a.partials[o] = 0.0;
+    if (a.keep) {
+      // read-only: 16 loads in flight per thread, the sum still in slot order
+      const double* in = a.partials_in + idx;
+      int sl = 0;
+      for (; sl + 16 <= ns; sl += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = in[(size_t)(sl + u) * total];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += v[u];
+      }
+      for (; sl < ns; ++sl) s += in[(size_t)sl * total];
+    } else {
+      for (int sl = 0; sl < ns; ++sl) {
+        size_t o = (size_t)sl * total + idx;
+        s += a.partials_in[o];
+        a.partials[o] = 0.0;
+      }
     }
     const int* perm = (q == 2) ? a.perm_k : a.perm_site;
     int dst = perm ? perm[j] : j;
@@ -546,7 +560,7 @@ hipError_t launch_gauss_init(const GaussArgs& a, int precision, int grid, hipStr
 
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {
   int total = kQ * a.n;
-  int block = 256;
+  int block = 64;  // one output per thread: small blocks spread the few waves over every CU
   int grid = (total + block - 1) / block;
   hipLaunchKernelGGL(reduce_kernel, dim3(grid), dim3(block), 0, s, a);
   return hipGetLastError();
