@@ -32,7 +32,8 @@ PEAK_FP32 = 157.3e12           # MI355X FP32 vector peak (FLOP/s)
 PEAK_BF16 = 2.5e15             # MI355X dense BF16 MFMA peak (FLOP/s)
 
 # square64: the headline workload (default).  lieb384: the disordered Lieb
-# chain with matrix hopping on MFMA, measured with --workload lieb384.
+# chain with matrix hopping (the banded kernel; --kernel mfma for the MFMA
+# product), measured with --workload lieb384.
 N_TRAJ = 16384
 FRAME_STEPS = 500
 FLOPS_PER_SITE_STEP = 150.0    # 2 FFTs x 5 N log2 N / N + 30 elementwise, N = 4096
@@ -57,7 +58,8 @@ WORKLOAD_TEXT = {
                 "incl. FP64 ensemble reduction",
     "lieb384": "1D Lieb chain 128 cells x 3 = 384 sites OBC, J=1 with bond disorder 0.1 and on-site "
                "disorder 0.2 (one realisation, seed 7), U=0.1 F=1.5 gamma=1 dt=0.005, 8192 trajectories, "
-               "hopping as a dense 384x384 BF16x3 MFMA GEMM; one step = one 200-step frame",
+               "hopping as the banded (half-width 2) stencil of the coupling matrix, or with --kernel mfma as a "
+               "dense 384x384 BF16x3 MFMA GEMM; one step = one 200-step frame",
 }
 
 
@@ -212,7 +214,7 @@ def main():
                     help="held-out points of the warm-start leg (square64) or evaluation (warmstart64)")
     ap.add_argument("--ws-traj", type=int, default=16384)
     ap.add_argument("--no-warmstart", action="store_true", help="square64: skip the warm-start leg")
-    ap.add_argument("--kernel", default="auto", choices=["auto", "stencil64", "fused64", "generic", "mfma"],
+    ap.add_argument("--kernel", default="auto", choices=["auto", "stencil64", "fused64", "generic", "mfma", "band"],
                     help="step kernel (auto: the fastest one that supports the workload)")
     ap.add_argument("--autotune", action="store_true", help="sweep512: autotune the stacking first")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -272,11 +274,19 @@ def main():
     site_steps = float(n_traj) * n_sites * fsteps * K
     value = site_steps / wall
     avg_ms = st["step_kernel_ms"] / max(st["step_launches"], 1)
-    if args.workload == "lieb384":
+    if args.workload == "lieb384" and info["kernel"] == "mfma":
         # executed MFMA work: 3 BF16 products of the real N x N hopping on Re and Im
         flops_site = 3 * 4.0 * n_sites
         peak, bound, kname = PEAK_BF16, "mfma", "mfma_step_kernel"
         basis = "executed BF16 MFMA flops: 3 products (hi.hi, hi.lo, lo.hi) x 4 N per site-step"
+    elif args.workload == "lieb384":
+        # the banded kernel: executed FP32 arithmetic per site-step, the
+        # (2W+1)-point complex hopping (W = 2 for the Lieb chain) and the
+        # drift / noise / update; its limiter is VALU issue (roofline.pmc)
+        flops_site = 4.0 * 5 + 30.0
+        peak, bound, kname = PEAK_FP32, "valu", info["kernel"] + "_step_kernel"
+        basis = ("executed FP32 flops: 4 (2W + 1) = 20 for the banded hopping (W = 2) + 30 for drift, noise "
+                 "scaling and update per site-step")
     else:
         # the workload's algorithmic work per site-step (the reference's FFT
         # path: 2 x 5 N log2 N / N + 30); the nearest-neighbour kernel evaluates
@@ -294,7 +304,7 @@ def main():
 
     traffic, counters = None, {}
     pmc = os.path.join(ROOT, "profiles", f"{info['kernel']}_64x64_16384traj_pmc.json" if args.workload == "square64"
-                       else "mfma_lieb384_8192traj_pmc.json")
+                       else f"{info['kernel']}_lieb384_8192traj_pmc.json")
     if os.path.exists(pmc):
         try:
             prof = json.load(open(pmc))

@@ -68,9 +68,10 @@ struct Ctx {
   int noise[5] = {0, 0, 0, 0, 0};
   // device-resident constants
   DevBuf d_base[5], d_amp[5], d_twx, d_twy, d_kcx, d_kcy, d_cscp, d_cscr, d_cscv, d_perm_site,
-      d_perm_k, d_ahi, d_alo, d_kamp, d_ft_times, d_ft_vals, d_ft_slopes;
+      d_perm_k, d_ahi, d_alo, d_kamp, d_ft_times, d_ft_vals, d_ft_slopes, d_band;
   int ft_n = 0;  // F(t) table times (0: static F)
   int np = 0;  // padded site count of the MFMA kernel
+  int band_w = -1;  // half-width of the coupling matrix (matrix hopping), -1: not banded
   // per-batch buffers
   DevBuf d_ids, d_beta, d_rsl, d_U, d_F, d_partials, d_obs, d_m2, d_m4, d_traj, d_prev;
   int csc_nnz = 0;
@@ -217,6 +218,7 @@ int choose_kernel(Ctx* c) {
   bool generic_ok = generic_pairs_per_thread(c->n, 256) > 0 && lds <= 160 * 1024 &&
                     (p.hopping == TWMC_HOP_MATRIX || (c->px.npass >= 0 && c->py.npass >= 0));
   bool mfma_ok = mfma_supported(c->n, p.hopping, c->precision, c->noisy_params ? 1 : 0);
+  const bool band_ok = band_supported(c->n, p.hopping, c->precision, c->noisy_params ? 1 : 0, c->band_w);
   const bool stencil_ok = stencil64_supported(p.nx, p.ny, p.hopping, c->precision,
                                              (c->noisy_params || !uniform) ? 1 : 0) && c->ft_n == 0;
   int k = p.kernel;
@@ -225,12 +227,19 @@ int choose_kernel(Ctx* c) {
                 "stencil 64x64 kernel needs nx = ny = 64, FFT hopping, FP32 and site-uniform, disorder-free parameters");
   // AUTO: the 64x64 nearest-neighbour form first (the same linear map as the
   // reference's FFT hopping, about twice as fast as the register FFT), then
-  // the register FFT, the MFMA matrix product, the LDS kernel
+  // the register FFT; for matrix hopping the banded stencil when the coupling
+  // is banded (a dense product would mostly move zeros), else the MFMA matrix
+  // product; the LDS kernel for everything else
   if (k == TWMC_KERNEL_AUTO)
     k = stencil_ok ? TWMC_KERNEL_STENCIL64
         : fused_ok ? TWMC_KERNEL_FUSED64
+        : band_ok  ? TWMC_KERNEL_BAND
         : mfma_ok  ? TWMC_KERNEL_MFMA
                    : TWMC_KERNEL_GENERIC;
+  if (k == TWMC_KERNEL_BAND && !band_ok)
+    return fail(c, TWMC_ERR_UNSUPPORTED,
+                "banded kernel needs matrix hopping with nonzeros at |i - j| <= 3 (cyclic when N = 64 S), FP32, at "
+                "most 512 sites and no per-trajectory disorder");
   if (k == TWMC_KERNEL_FUSED64 && !fused_ok)
     return fail(c, TWMC_ERR_UNSUPPORTED,
                 "fused 64x64 kernel needs nx = ny = 64, FFT hopping, FP32 and site-uniform, disorder-free parameters");
@@ -295,6 +304,44 @@ int advance_stencil(Ctx* c, const StepArgs& a, hipStream_t s) {
     HIPCHK(c, launch_stencil64(a, c->grid, s));
     HIPCHK(c, timed(c, c->ev_step, s, false, e0));
     c->stats.site_steps += (uint64_t)a.n_steps * (uint64_t)n * c->n_traj;
+  }
+  return TWMC_OK;
+}
+
+// banded segment: at a capture the site-major sums read the frame state
+// first, then the step launch records the per-trajectory means (and the
+// dump) before stepping, then the reduction
+int advance_band(Ctx* c, const StepArgs& a, hipStream_t s) {
+  const int n = c->n;
+  int chunks = 0;
+  hipEvent_t r0 = nullptr, e0 = nullptr;
+  if (a.capture) {
+    HIPCHK(c, timed(c, c->ev_red, s, true, r0));
+    SiteSumArgs ss{};
+    ss.beta = a.beta;
+    ss.partials = static_cast<double*>(c->d_partials.p);
+    ss.n = n;
+    ss.n_traj = (int)c->n_traj;
+    ss.zero_k = 1;
+    chunks = site_sum_chunks(c->n_traj, kSiteSumChunks, &ss.chunk);
+    HIPCHK(c, launch_site_sums(ss, c->precision, s));
+    HIPCHK(c, timed(c, c->ev_red, s, false, r0));
+  }
+  HIPCHK(c, timed(c, c->ev_step, s, true, e0));
+  HIPCHK(c, launch_band(a, c->grid, s));
+  HIPCHK(c, timed(c, c->ev_step, s, false, e0));
+  c->stats.site_steps += (uint64_t)a.n_steps * (uint64_t)n * c->n_traj;
+  if (a.capture) {
+    ReduceArgs r{};
+    r.n = n;
+    r.slots = chunks;
+    r.keep = 1;
+    r.partials_in = static_cast<const double*>(c->d_partials.p);
+    r.partials = static_cast<double*>(c->d_partials.p);
+    r.obs = static_cast<double*>(c->d_obs.p) + (size_t)a.frame * kQ * n;
+    HIPCHK(c, timed(c, c->ev_red, s, true, r0));
+    HIPCHK(c, launch_reduce(r, s));
+    HIPCHK(c, timed(c, c->ev_red, s, false, r0));
   }
   return TWMC_OK;
 }
@@ -380,6 +427,7 @@ int twmc_create(const twmc_params* p, int device, void** ctx_out) {
     c->px.len = c->py.len = 0;
     c->px.npass = c->py.npass = 0;
   }
+  if (p->hopping == TWMC_HOP_MATRIX) c->band_w = band_width(p->coupling, c->n);
   int rc = choose_kernel(c);
   if (rc != TWMC_OK) {
     std::string m = c->err;
@@ -488,6 +536,16 @@ int twmc_create(const twmc_params* p, int device, void** ctx_out) {
     if (ee == hipSuccess) ee = hipStreamSynchronize(s);
     if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "upload MFMA operand");
   }
+  if (fail_rc == TWMC_OK && c->kernel == TWMC_KERNEL_BAND) {
+    // dt C_(j+d) j per site j and offset d, padded to 64 S sites
+    c->band_w = std::max(1, c->band_w);
+    const int S = band_sites_per_lane(c->n);
+    std::vector<float> coef((size_t)64 * S * (2 * c->band_w + 1));
+    band_coefficients(p->coupling, c->n, c->band_w, p->dt, coef.data());
+    hipError_t ee = upload(c->d_band, coef.data(), coef.size() * sizeof(float), s);
+    if (ee == hipSuccess) ee = hipStreamSynchronize(s);
+    if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "upload band coefficients");
+  }
   if (fail_rc == TWMC_OK && c->kernel == TWMC_KERNEL_FUSED64) {
     std::vector<int> ps(4096), pk(4096);
     fused64_perms(ps.data(), pk.data());
@@ -521,10 +579,11 @@ void twmc_destroy(void* ctx) {
     (void)hipEventDestroy(pr.first);
     (void)hipEventDestroy(pr.second);
   }
-  DevBuf* bufs[] = {&c->d_twx, &c->d_twy, &c->d_kcx, &c->d_kcy, &c->d_cscp, &c->d_cscr,
-                    &c->d_cscv, &c->d_perm_site, &c->d_perm_k, &c->d_ahi, &c->d_alo, &c->d_kamp, &c->d_ft_times, &c->d_ft_vals, &c->d_ft_slopes, &c->d_ids, &c->d_beta, &c->d_rsl,
-                    &c->d_U, &c->d_F, &c->d_partials, &c->d_obs, &c->d_m2, &c->d_m4, &c->d_traj,
-                    &c->d_prev};
+  DevBuf* bufs[] = {&c->d_twx,       &c->d_twy,      &c->d_kcx,       &c->d_kcy,  &c->d_cscp, &c->d_cscr,
+                    &c->d_cscv,      &c->d_perm_site, &c->d_perm_k,   &c->d_ahi,  &c->d_alo,  &c->d_kamp,
+                    &c->d_ft_times,  &c->d_ft_vals,  &c->d_ft_slopes, &c->d_band, &c->d_ids,  &c->d_beta,
+                    &c->d_rsl,       &c->d_U,        &c->d_F,         &c->d_partials, &c->d_obs, &c->d_m2,
+                    &c->d_m4,        &c->d_traj,     &c->d_prev};
   for (DevBuf* b : bufs) b->release();
   for (int k = 0; k < 5; ++k) {
     c->d_base[k].release();
@@ -587,6 +646,10 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
     if (cap_per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "capture kernel does not fit on a CU");
     c->cap_grid = (int)std::min<size_t>(B, (size_t)cap_per_cu * cus);
     HIPCHK(c, stencil64_occupancy(&per_cu));
+  } else if (c->kernel == TWMC_KERNEL_BAND) {
+    c->block = 64;
+    c->lds = 0;
+    HIPCHK(c, band_occupancy(c->n, c->band_w, &per_cu));
   } else if (c->kernel == TWMC_KERNEL_MFMA) {
     c->block = 256;
     c->lds = mfma_lds_bytes(c->np);
@@ -598,7 +661,7 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
     HIPCHK(c, generic_occupancy(c->precision, n, c->block, c->lds, &per_cu));
   }
   if (per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");
-  if (c->kernel == TWMC_KERNEL_STENCIL64 || c->kernel == TWMC_KERNEL_FUSED64)
+  if (c->kernel == TWMC_KERNEL_STENCIL64 || c->kernel == TWMC_KERNEL_FUSED64 || c->kernel == TWMC_KERNEL_BAND)
     c->grid = balanced_wave_grid(B, per_cu, cus);
   else
     c->grid = (int)std::min<size_t>(units, (size_t)per_cu * cus);
@@ -742,8 +805,13 @@ int twmc_batch_advance(void* ctx, uint64_t n_segments) {
       a.ft_times = static_cast<const double*>(c->d_ft_times.p);
       a.ft_vals = c->d_ft_vals.p;
       a.ft_slopes = c->d_ft_slopes.p;
+      a.band_coef = static_cast<const float*>(c->d_band.p);
+      a.band_w = c->band_w;
       if (c->kernel == TWMC_KERNEL_STENCIL64) {
         int rc = advance_stencil(c, a, s);
+        if (rc != TWMC_OK) return rc;
+      } else if (c->kernel == TWMC_KERNEL_BAND) {
+        int rc = advance_band(c, a, s);
         if (rc != TWMC_OK) return rc;
       } else if (a.n_steps > 0 || a.capture) {
         hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -420,6 +420,7 @@ __global__ void __launch_bounds__(256) site_sums_kernel(SiteSumArgs a) {
   part[a.n + i] = s4;
   part[3 * a.n + i] = sr;
   part[4 * a.n + i] = si;
+  if (a.zero_k) part[2 * a.n + i] = 0.0;
 }
 
 __global__ void reduce_kernel(ReduceArgs a) {

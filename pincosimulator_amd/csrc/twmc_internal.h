@@ -83,6 +83,8 @@ struct StepArgs {
   const void* a_hi;
   const void* a_lo;
   int np;                    // n padded to a multiple of 32
+  const float* band_coef;    // banded kernel: [64 S][2W + 1] dt C_(j+d) j
+  int band_w;                // banded kernel: half-width W
   // time-dependent drive: F(t) = ft_vals[id] + (t - ft_times[id]) ft_slopes[id]
   // with id the last table time below t (F = ft_vals[0] up to ft_times[0]);
   // t = t0 + s dt before step s of this launch, in FP64
@@ -155,6 +157,7 @@ struct SiteSumArgs {
   const void* beta;  // [n_traj][n] complex, precision of the batch
   double* partials;  // [chunks][kQ][n]
   int n, n_traj, chunk;
+  int zero_k;        // 1: also write zeros to row 2 (no k-space sums)
 };
 hipError_t launch_site_sums(const SiteSumArgs& a, int precision, hipStream_t s);
 // chunks of at most max_chunks for n_traj trajectories; *chunk = trajectories per chunk
@@ -186,6 +189,17 @@ bool mfma_supported(int n, int hopping, int precision, int param_stride);
 int mfma_padded(int n);
 size_t mfma_lds_bytes(int np);
 hipError_t launch_mfma(const StepArgs& a, int grid, hipStream_t s);
+
+// banded coupling (matrix hopping with every nonzero C_ij at |i - j| <= W,
+// cyclic when N = 64 S): one wave per trajectory, S sites per lane
+constexpr int kBandMaxSites = 512;
+constexpr int kBandMaxWidth = 3;
+int band_sites_per_lane(int n);
+int band_width(const double* coupling, int n);  // W, or -1 when n is out of range
+bool band_supported(int n, int hopping, int precision, int param_stride, int width);
+void band_coefficients(const double* coupling, int n, int w, double dt, float* out);  // [64 S][2w+1]
+hipError_t launch_band(const StepArgs& a, int grid, hipStream_t s);
+hipError_t band_occupancy(int n, int w, int* blocks_per_cu);
 hipError_t mfma_occupancy(int np, int* blocks_per_cu);
 // splits A (row-major np x np, FP64) into BF16 hi and lo parts (raw 16-bit)
 void mfma_split(const double* a, size_t count, uint16_t* hi, uint16_t* lo);
