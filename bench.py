@@ -344,6 +344,13 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         log(f"bench: CPU baseline on {threads} threads")
         out["cpu_baseline"] = cpu_baseline(threads, base)
+        if threads > 1:
+            # the same sample on one thread per physical core (2-way SMT)
+            half = threads // 2
+            log(f"bench: CPU baseline on {half} threads")
+            b = cpu_baseline(half, base, target_s=8.0)
+            out["cpu_baseline"]["per_physical_core_run"] = {"value": b["value"], "cores": half,
+                                                            "sample": b["sample"]}
     print(json.dumps(out), flush=True)
 
 
