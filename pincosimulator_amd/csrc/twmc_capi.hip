@@ -805,6 +805,9 @@ int twmc_batch_advance(void* ctx, uint64_t n_segments) {
       a.ft_times = static_cast<const double*>(c->d_ft_times.p);
       a.ft_vals = c->d_ft_vals.p;
       a.ft_slopes = c->d_ft_slopes.p;
+      a.u_real = 1;
+      for (int i = 0; i < n && a.u_real; ++i) a.u_real = c->base[1][2 * i + 1] == 0.0;
+      if (c->noise[1] != TWMC_NOISE_NONE) a.u_real = 0;
       a.band_coef = static_cast<const float*>(c->d_band.p);
       a.band_w = c->band_w;
       if (c->kernel == TWMC_KERNEL_STENCIL64) {

@@ -83,6 +83,7 @@ struct StepArgs {
   const void* a_hi;
   const void* a_lo;
   int np;                    // n padded to a multiple of 32
+  int u_real;                // 1: Im U = 0 at every site, in every trajectory
   const float* band_coef;    // banded kernel: [64 S][2W + 1] dt C_(j+d) j
   int band_w;                // banded kernel: half-width W
   // time-dependent drive: F(t) = ft_vals[id] + (t - ft_times[id]) ft_slopes[id]

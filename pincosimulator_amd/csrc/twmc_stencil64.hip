@@ -38,6 +38,9 @@ __device__ __forceinline__ float rot_down(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x13c, 0xf, 0xf, true));
 }
 
+// UI0: Im U = 0 (a real interaction), so the real part of the drift factor is
+// the constant A0 and its per-site FMA drops out
+template <bool UI0>
 __global__ void __launch_bounds__(64, 2) stencil64_step_kernel(StepArgs g) {
   const int lane = threadIdx.x;  // row x
 
@@ -96,7 +99,7 @@ __global__ void __launch_bounds__(64, 2) stencil64_step_kernel(StepArgs g) {
             syi = v.im + y2.im;
           }
           const float m = fmaf(v.im, v.im, v.re * v.re);
-          const float cr = fmaf(-Ui, m, A0), ci = fmaf(Ur, m, B0);
+          const float cr = UI0 ? A0 : fmaf(-Ui, m, A0), ci = fmaf(Ur, m, B0);
           // increment = noise + dt iF + drift + hopping, all small, added to beta once
           float dr = fmaf(rs[h], cs[h], fr);
           dr = fmaf(cr, v.re, dr);
@@ -124,12 +127,15 @@ bool stencil64_supported(int nx, int ny, int hopping, int precision, int param_s
 }
 
 hipError_t launch_stencil64(const StepArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(stencil64_step_kernel, dim3(grid), dim3(64), 0, s, a);
+  if (a.u_real)
+    hipLaunchKernelGGL(stencil64_step_kernel<true>, dim3(grid), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(stencil64_step_kernel<false>, dim3(grid), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t stencil64_occupancy(int* blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)stencil64_step_kernel, 64, 0);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)stencil64_step_kernel<false>, 64, 0);
 }
 
 }  // namespace twmc

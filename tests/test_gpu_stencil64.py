@@ -68,3 +68,19 @@ def test_stencil_observables_vs_fp64():
     g64, _ = observables.g2_jackknife(r64)
     assert np.all(np.abs(n32 - n64) < 3 * se64 + 1e-12)
     assert np.all(np.abs(g32[1:] - g64[1:]) < 3 * gse[1:])
+
+
+def test_stencil_complex_interaction_vs_fp64_oracle():
+    """Im U != 0 (two-body loss) takes the kernel variant that keeps the
+    |beta|^2-dependent real part of the drift factor."""
+    spec = c2_spec(t_end=0.8, frame_steps=200, U=0.05 + 0.02j)
+    ids = np.array([2, 9])
+    print("stencil64 complex U vs oracle", flush=True)
+    eng = Engine(system(spec), precision=32, kernel="stencil64")
+    res = eng.run(ids, traj=True)
+    eng.close()
+    for j, t in enumerate(ids):
+        fr, _, _ = oracle.run_trajectory(dict(spec, lattice=oracle.LAT_FFT), int(t))
+        err = rel_err(res.traj[j], fr)
+        print(f"  traj {t}: {err:.3e}")
+        assert err < 2e-4
