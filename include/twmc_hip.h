@@ -72,7 +72,8 @@ enum {
   TWMC_KERNEL_FUSED64 = 2,  /* register-resident 64x64 FP32 kernel */
   TWMC_KERNEL_MFMA = 3,     /* matrix hopping as BF16x3 MFMA GEMM (FP32 state) */
   TWMC_KERNEL_STENCIL64 = 4, /* 64x64 FP32, hopping as its exact nearest-neighbour form */
-  TWMC_KERNEL_BAND = 5       /* matrix hopping of half-width <= 3, FP32, <= 512 sites, shared parameters */
+  TWMC_KERNEL_BAND = 5,      /* matrix hopping of half-width <= 3, FP32, <= 512 sites, shared parameters */
+  TWMC_KERNEL_STACK = 6      /* FFT hopping, FP32, site-uniform: stacked-trajectory LDS stencil (<= 4096 sites) */
 };
 /* output mask bits for twmc_batch_desc.outputs */
 enum {

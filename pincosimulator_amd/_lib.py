@@ -19,8 +19,8 @@ NOISE_NONE, NOISE_UNIFORM, NOISE_GAUSSIAN = 0, 1, 2
 HOP_FFT, HOP_MATRIX = 0, 1
 ONSITE_OMEGA, ONSITE_DETUNING_MINUS_E = 0, 1
 INIT_SETTINGS, INIT_FIXEDPOINT, INIT_PREVIOUS, INIT_GAUSSIAN = 0, 1, 2, 3
-KERNEL_AUTO, KERNEL_GENERIC, KERNEL_FUSED64, KERNEL_MFMA, KERNEL_STENCIL64, KERNEL_BAND = 0, 1, 2, 3, 4, 5
-KERNEL_NAMES = {0: "auto", 1: "generic", 2: "fused64", 3: "mfma", 4: "stencil64", 5: "band"}
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_FUSED64, KERNEL_MFMA, KERNEL_STENCIL64, KERNEL_BAND, KERNEL_STACK = range(7)
+KERNEL_NAMES = {0: "auto", 1: "generic", 2: "fused64", 3: "mfma", 4: "stencil64", 5: "band", 6: "stack"}
 OUT_OBS, OUT_TRAJ, OUT_FINAL = 1, 2, 4
 
 # every symbol include/twmc_hip.h declares

@@ -72,6 +72,7 @@ struct Ctx {
   int ft_n = 0;  // F(t) table times (0: static F)
   int np = 0;  // padded site count of the MFMA kernel
   int band_w = -1;  // half-width of the coupling matrix (matrix hopping), -1: not banded
+  StackPlan stack;  // TWMC_KERNEL_STACK: the batch's point and work items on the sweep kernel
   // per-batch buffers
   DevBuf d_ids, d_beta, d_rsl, d_U, d_F, d_partials, d_obs, d_m2, d_m4, d_traj, d_prev;
   int csc_nnz = 0;
@@ -219,6 +220,8 @@ int choose_kernel(Ctx* c) {
                     (p.hopping == TWMC_HOP_MATRIX || (c->px.npass >= 0 && c->py.npass >= 0));
   bool mfma_ok = mfma_supported(c->n, p.hopping, c->precision, c->noisy_params ? 1 : 0);
   const bool band_ok = band_supported(c->n, p.hopping, c->precision, c->noisy_params ? 1 : 0, c->band_w);
+  const bool stack_ok = p.hopping == TWMC_HOP_FFT && c->precision == 32 && !c->noisy_params && uniform &&
+                        c->ft_n == 0 && stack_supported(p.nx, p.ny) && generic_ok;
   const bool stencil_ok = stencil64_supported(p.nx, p.ny, p.hopping, c->precision,
                                              (c->noisy_params || !uniform) ? 1 : 0) && c->ft_n == 0;
   int k = p.kernel;
@@ -227,15 +230,21 @@ int choose_kernel(Ctx* c) {
                 "stencil 64x64 kernel needs nx = ny = 64, FFT hopping, FP32 and site-uniform, disorder-free parameters");
   // AUTO: the 64x64 nearest-neighbour form first (the same linear map as the
   // reference's FFT hopping, about twice as fast as the register FFT), then
-  // the register FFT; for matrix hopping the banded stencil when the coupling
+  // the register FFT; other uniform FP32 lattices on the stacked LDS stencil;
+  // for matrix hopping the banded stencil when the coupling
   // is banded (a dense product would mostly move zeros), else the MFMA matrix
   // product; the LDS kernel for everything else
   if (k == TWMC_KERNEL_AUTO)
     k = stencil_ok ? TWMC_KERNEL_STENCIL64
         : fused_ok ? TWMC_KERNEL_FUSED64
+        : stack_ok ? TWMC_KERNEL_STACK
         : band_ok  ? TWMC_KERNEL_BAND
         : mfma_ok  ? TWMC_KERNEL_MFMA
                    : TWMC_KERNEL_GENERIC;
+  if (k == TWMC_KERNEL_STACK && !stack_ok)
+    return fail(c, TWMC_ERR_UNSUPPORTED,
+                "stacked kernel needs FFT hopping, FP32, site-uniform disorder-free parameters, static F, an even "
+                "row length <= 512, at most 64 rows and 4096 sites");
   if (k == TWMC_KERNEL_BAND && !band_ok)
     return fail(c, TWMC_ERR_UNSUPPORTED,
                 "banded kernel needs matrix hopping with nonzeros at |i - j| <= 3 (cyclic when N = 64 S), FP32, at "
@@ -342,6 +351,34 @@ int advance_band(Ctx* c, const StepArgs& a, hipStream_t s) {
     HIPCHK(c, timed(c, c->ev_red, s, true, r0));
     HIPCHK(c, launch_reduce(r, s));
     HIPCHK(c, timed(c, c->ev_red, s, false, r0));
+  }
+  return TWMC_OK;
+}
+
+// stacked segment: a frame is the LDS kernel's capture-only launch and the
+// reduction (the per-site and per-mode sums), then the steps on the sweep kernel
+int advance_stack(Ctx* c, const StepArgs& a, hipStream_t s) {
+  const int n = c->n;
+  hipEvent_t r0 = nullptr, e0 = nullptr;
+  if (a.capture) {
+    StepArgs ca = a;
+    ca.n_steps = 0;
+    HIPCHK(c, timed(c, c->ev_red, s, true, r0));
+    HIPCHK(c, launch_generic(ca, c->precision, c->grid, c->block, c->lds, s));
+    ReduceArgs r{};
+    r.n = n;
+    r.slots = c->grid;
+    r.partials_in = static_cast<const double*>(c->d_partials.p);
+    r.partials = static_cast<double*>(c->d_partials.p);
+    r.obs = static_cast<double*>(c->d_obs.p) + (size_t)a.frame * kQ * n;
+    HIPCHK(c, launch_reduce(r, s));
+    HIPCHK(c, timed(c, c->ev_red, s, false, r0));
+  }
+  if (a.n_steps > 0) {
+    HIPCHK(c, timed(c, c->ev_step, s, true, e0));
+    HIPCHK(c, stack_launch(c->stack, a, s));
+    HIPCHK(c, timed(c, c->ev_step, s, false, e0));
+    c->stats.site_steps += (uint64_t)a.n_steps * (uint64_t)n * c->n_traj;
   }
   return TWMC_OK;
 }
@@ -571,6 +608,7 @@ void twmc_destroy(void* ctx) {
   Ctx* c = static_cast<Ctx*>(ctx);
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  stack_release(c->stack);
   for (auto& pr : c->ev_step) {
     (void)hipEventDestroy(pr.first);
     (void)hipEventDestroy(pr.second);
@@ -665,6 +703,22 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
     c->grid = balanced_wave_grid(B, per_cu, cus);
   else
     c->grid = (int)std::min<size_t>(units, (size_t)per_cu * cus);
+  if (c->kernel == TWMC_KERNEL_STACK) {
+    // steps on the sweep kernel; frames stay with the LDS kernel (geometry above).
+    // The site-uniform constants, formed as the init kernel forms them
+    const double orr = c->base[0][0], oi = c->base[0][1];
+    double rsl[2];
+    if (c->p.onsite == TWMC_ONSITE_OMEGA) {
+      rsl[0] = oi - c->p.gamma / 2.0;  // rsl = -i omega - gamma/2 (TWMCBaseSolver.cpp:74)
+      rsl[1] = -orr;
+    } else {
+      const double dr = -orr + c->p.detuning, di = -oi;  // TWMCLiebSolver.cpp:44-45
+      rsl[0] = di - c->p.gamma / 2.0;
+      rsl[1] = -dr;
+    }
+    HIPCHK(c, stack_prepare(c->stack, c->p.nx, c->p.ny, (int)B, rsl, c->base[1].data(), c->base[2].data(), c->dt4,
+                            -2.0 * c->p.J));
+  }
 
   size_t sets = c->noisy_params ? B : 1;
   HIPCHK(c, upload(c->d_ids, d->traj_ids, B * sizeof(uint64_t), s));
@@ -815,6 +869,9 @@ int twmc_batch_advance(void* ctx, uint64_t n_segments) {
         if (rc != TWMC_OK) return rc;
       } else if (c->kernel == TWMC_KERNEL_BAND) {
         int rc = advance_band(c, a, s);
+        if (rc != TWMC_OK) return rc;
+      } else if (c->kernel == TWMC_KERNEL_STACK) {
+        int rc = advance_stack(c, a, s);
         if (rc != TWMC_OK) return rc;
       } else if (a.n_steps > 0 || a.capture) {
         hipEvent_t e0 = nullptr, e1 = nullptr;

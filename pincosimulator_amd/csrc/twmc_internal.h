@@ -191,6 +191,20 @@ int mfma_padded(int n);
 size_t mfma_lds_bytes(int np);
 hipError_t launch_mfma(const StepArgs& a, int grid, hipStream_t s);
 
+// one-point batches of the main API on the sweep kernel: FP32, FFT hopping,
+// site-uniform shared parameters, no F(t)
+struct StackPlan {
+  void* d_point = nullptr;
+  void* d_items = nullptr;
+  void* d_counter = nullptr;
+  int n_items = 0, grid = 0, tb = 1;
+};
+bool stack_supported(int nx, int ny);
+hipError_t stack_prepare(StackPlan& sp, int nx, int ny, int n_traj, const double rsl[2], const double u[2],
+                         const double f[2], double dt4, double kscale);
+hipError_t stack_launch(const StackPlan& sp, const StepArgs& a, hipStream_t s);
+void stack_release(StackPlan& sp);
+
 // banded coupling (matrix hopping with every nonzero C_ij at |i - j| <= W,
 // cyclic when N = 64 S): one wave per trajectory, S sites per lane
 constexpr int kBandMaxSites = 512;

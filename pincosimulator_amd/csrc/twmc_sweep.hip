@@ -77,6 +77,7 @@ struct SweepArgs {
   uint32_t key0, key1;
   uint64_t step0;
   int n_steps, capture, frame, n_frames;
+  const uint64_t* traj_ids;  // one-point batches of the main API: ids by index (else traj_id0 + index)
 };
 
 // clears the work-queue counters of the capture and the step launch
@@ -115,7 +116,7 @@ __device__ __forceinline__ void step_item(const SweepArgs& g, const SweepPointDe
     up[j] = ((r0 + (r == 0 ? nxt - 1 : r - 1)) * ny + col) / 2;  // pair index
     dn[j] = ((r0 + (r + 1 == nxt ? 0 : r + 1)) * ny + col) / 2;
     pid[j] = (uint32_t)(p - tr * (nxt * ny / 2));
-    trj[j] = traj0 + (uint64_t)tr;
+    trj[j] = g.traj_ids ? g.traj_ids[item.t0 + tr] : traj0 + (uint64_t)tr;
   }
   // step constants: dt (rsl + i U (|b|^2 - 1)) = (A0 - Ui |b|^2) + i (B0 + Ur |b|^2),
   // dt iF = (fr, fi), dt h = -i jdt S, sigma^2 folded into the Box-Muller log multiplier
@@ -545,6 +546,84 @@ int init_state(SweepCtx* c) {
 }
 
 }  // namespace
+
+namespace twmc {
+
+// The main API's FP32 FFT-hopping batches of one site-uniform system run
+// their steps on this kernel as a single point: the state layout is the
+// same (trajectories contiguous, each row-major), each step item stacks tb
+// trajectories, and the trajectory ids come from the batch's id array.  The
+// batch's frames stay with the LDS kernel (its capture-only launch), which
+// forms the per-site and per-mode sums of that API.
+bool stack_supported(int nx, int ny) {
+  return ny % 2 == 0 && ny <= kMaxNy && nx <= kMaxNx && nx * ny <= kSweepMaxSites && (nx == 1 || ny > 1);
+}
+
+hipError_t stack_prepare(StackPlan& sp, int nx, int ny, int n_traj, const double rsl[2], const double u[2],
+                         const double f[2], double dt4, double kscale) {
+  stack_release(sp);
+  SweepPointDev P{};
+  P.nx = nx;
+  P.ny = ny;
+  P.fft_x = nx > 1 ? 1 : 0;
+  P.tb = std::max(1, std::min({n_traj, 2048 / (nx * ny), kMaxRows}));
+  P.rsl_re = float(rsl[0]);
+  P.rsl_im = float(rsl[1]);
+  P.u_re = float(u[0]);
+  P.u_im = float(u[1]);
+  P.f_re = float(f[0]);
+  P.f_im = float(f[1]);
+  P.dt4 = float(dt4);
+  P.kscale = float(kscale);
+  P.n_traj = n_traj;
+  std::vector<SweepItemDev> items;
+  for (int t0 = 0; t0 < n_traj; t0 += P.tb) items.push_back(SweepItemDev{0, t0});
+  sp.n_items = (int)items.size();
+  sp.tb = P.tb;
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sweep_step_kernel<false>, kSweepBlock,
+                                                     sweep_lds_bytes(false));
+  sp.grid = std::max(1, std::min(sp.n_items, std::max(1, per_cu) * cus));
+  if (e == hipSuccess) e = hipMalloc(&sp.d_point, sizeof(SweepPointDev));
+  if (e == hipSuccess) e = hipMalloc(&sp.d_items, items.size() * sizeof(SweepItemDev));
+  if (e == hipSuccess) e = hipMalloc(&sp.d_counter, 2 * sizeof(unsigned int));
+  if (e == hipSuccess) e = hipMemcpy(sp.d_point, &P, sizeof(SweepPointDev), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(sp.d_items, items.data(), items.size() * sizeof(SweepItemDev), hipMemcpyHostToDevice);
+  if (e != hipSuccess) stack_release(sp);
+  return e;
+}
+
+hipError_t stack_launch(const StackPlan& sp, const StepArgs& a, hipStream_t s) {
+  SweepArgs g{};
+  g.points = (const SweepPointDev*)sp.d_point;
+  g.items = (const SweepItemDev*)sp.d_items;
+  g.n_items = sp.n_items;
+  g.counter = (unsigned int*)sp.d_counter;
+  g.beta = (cpx<float>*)a.beta;
+  g.dt = float(a.dt);
+  g.key0 = a.key0;
+  g.key1 = a.key1;
+  g.step0 = a.step0;
+  g.n_steps = a.n_steps;
+  g.traj_ids = a.traj_ids;
+  hipLaunchKernelGGL(sweep_reset_kernel, dim3(1), dim3(64), 0, s, (unsigned int*)sp.d_counter);
+  hipLaunchKernelGGL(sweep_step_kernel<false>, dim3(sp.grid), dim3(kSweepBlock), sweep_lds_bytes(false), s, g);
+  return hipGetLastError();
+}
+
+void stack_release(StackPlan& sp) {
+  void* bufs[] = {sp.d_point, sp.d_items, sp.d_counter};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  sp.d_point = sp.d_items = sp.d_counter = nullptr;
+  sp.n_items = 0;
+}
+
+}  // namespace twmc
 
 extern "C" {
 

@@ -18,7 +18,8 @@ from . import _lib
 from ._lib import TWMCError, check
 
 _KERNELS = {"auto": _lib.KERNEL_AUTO, "generic": _lib.KERNEL_GENERIC, "fused64": _lib.KERNEL_FUSED64,
-            "mfma": _lib.KERNEL_MFMA, "stencil64": _lib.KERNEL_STENCIL64, "band": _lib.KERNEL_BAND}
+            "mfma": _lib.KERNEL_MFMA, "stencil64": _lib.KERNEL_STENCIL64, "band": _lib.KERNEL_BAND,
+            "stack": _lib.KERNEL_STACK}
 _INITS = {"settings": _lib.INIT_SETTINGS, "fixedpoint": _lib.INIT_FIXEDPOINT,
           "previous": _lib.INIT_PREVIOUS, "gaussian": _lib.INIT_GAUSSIAN}
 
