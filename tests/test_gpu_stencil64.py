@@ -1,7 +1,7 @@
 """64x64 FP32 kernel with the hopping in its exact nearest-neighbour form,
 checked against the FP64 oracle (which runs the reference's FFT form) per
 trajectory, against the FFT kernel, and through its ensemble sums (captured by
-the FFT kernel's capture launch)."""
+the register-FFT capture kernel and the site-major sums kernel)."""
 import numpy as np
 import pytest
 
