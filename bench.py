@@ -134,6 +134,12 @@ def bench_sweep(args):
     value = site_steps / wall
     log(f"bench: sweep {args.steps} frames: wall {wall:.3f} s, graph kernel time {out['kernel_ms']:.1f} ms, "
         f"{value:.4e} site-steps/s")
+    # algorithmic work of each point's lattice (the reference's FFT hopping,
+    # 2 x 5 N log2 N / N + 30 per site-step) over the graph's kernel time
+    import math
+    flops = sum(p["n_traj"] * p["nx"] * p["ny"] * (10.0 * math.log2(p["nx"] * p["ny"]) + 30.0)
+                for p in pts) * 500.0 * args.steps
+    achieved = flops / (out["kernel_ms"] * 1e-3)
     res = {
         "metric": METRIC, "value": value, "unit": "trajectory·site-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
@@ -143,6 +149,12 @@ def bench_sweep(args):
                                "one persistent launch per 500-step frame, hipGraph frame loop",
                    "stacking": {f"{pts[k]['nx']}x{pts[k]['ny']}": tb[k] for k in range(9)},
                    "kernel_ms": out["kernel_ms"]},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32, "traffic": None, "kernel": "sweep_step_kernel<false>",
+                     "avg_launch_ms": out["kernel_ms"] / args.steps,
+                     "flops_basis": "algorithmic: 10 log2 N + 30 flops per site-step of each point's lattice "
+                                    "(the reference's FFT hopping), over the graph's kernel time per frame "
+                                    "(step launch plus the frame's capture launch)"},
     }
     sw.close()
     print(json.dumps(res), flush=True)
