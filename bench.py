@@ -122,7 +122,10 @@ def bench_sweep(args):
     from pincosimulator_amd import Sweep
     pts = sweep_points()
     sw = Sweep(pts, 0.002, 500, 20260103)
-    tb = sw.autotune(steps=50) if args.autotune else sw.batching()
+    t_at = time.perf_counter()
+    tb = sw.batching() if args.no_autotune else sw.autotune(steps=50)
+    if not args.no_autotune:
+        log(f"bench: autotuned the stacking in {time.perf_counter() - t_at:.1f} s (outside the timed region)")
     log(f"bench: sweep stacking per shape: " + ", ".join(f"{pts[k]['nx']}x{pts[k]['ny']}:{tb[k]}" for k in range(9)))
     if args.warmup:
         sw.run(args.warmup * 500 * 0.002, graph=True)
@@ -148,6 +151,7 @@ def bench_sweep(args):
                                "{1D 32..512, 2D 16..64}, 128 trajectories each, U=0.05 gamma=1 dt=0.002, "
                                "one persistent launch per 500-step frame, hipGraph frame loop",
                    "stacking": {f"{pts[k]['nx']}x{pts[k]['ny']}": tb[k] for k in range(9)},
+                   "autotuned": not args.no_autotune,
                    "kernel_ms": out["kernel_ms"]},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32, "traffic": None, "kernel": "sweep_step_kernel<false>",
@@ -250,7 +254,8 @@ def main():
     ap.add_argument("--no-warmstart", action="store_true", help="square64: skip the warm-start leg")
     ap.add_argument("--kernel", default="auto", choices=["auto", "stencil64", "fused64", "generic", "mfma", "band"],
                     help="step kernel (auto: the fastest one that supports the workload)")
-    ap.add_argument("--autotune", action="store_true", help="sweep512: autotune the stacking first")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="sweep512: keep the default stacking instead of autotuning it first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 256 MiB device-to-device copy (torch) whose HBM counters "
                          "calibrate FETCH_SIZE / WRITE_SIZE in a --pmc profile")
