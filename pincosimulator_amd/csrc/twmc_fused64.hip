@@ -104,10 +104,12 @@ __global__ void __launch_bounds__(64, 1) fused64_step_kernel(StepArgs g) {
   const cpx<float> u0 = reinterpret_cast<const cpx<float>*>(g.U)[0];
   const cpx<float> f0 = reinterpret_cast<const cpx<float>*>(g.F)[0];
   const float dtf = float(g.dt);
-  const float A = dtf * r0.re, Bc = dtf * r0.im;          // dt * rsl
   const float Ur = dtf * u0.re, Ui = dtf * u0.im;         // dt * U
+  // dt (rsl + i U (|b|^2 - 1)) = (A0 - Ui |b|^2) + i (B0 + Ur |b|^2)
+  const float A0 = fmaf(dtf, r0.re, Ui), B0 = fmaf(dtf, r0.im, -Ur);
   const float fr = -dtf * f0.im, fi = dtf * f0.re;        // dt * i F
   const float sig = float(g.dt4);
+  const float nl = -1.3862943611198906f * sig * sig;      // -2 ln2 sigma^2 (Box-Muller)
   // K = i kscale (cos kx + cos ky); kscale dt / N is applied in the update
   const float kfac = float(g.kscale * g.dt * g.inv_n);
   const float cy = cosf(6.283185307179586f * float(rev8(lane)) / 64.0f);
@@ -205,18 +207,25 @@ __global__ void __launch_bounds__(64, 1) fused64_step_kernel(StepArgs g) {
       for (int p = 0; p < kL / 2; ++p) {
         const u32x4 w = philox4x32_10(make_counter(traj, TAG_WIENER, step, pbase + (uint32_t)p),
                                       g.key0, g.key1);
-        float xr0, xi0, xr1, xi1;
-        box_muller_f32(w.x, w.y, xr0, xi0);
-        box_muller_f32(w.z, w.w, xr1, xi1);
+        float rs[2], cs[2], sn[2];
+        box_muller_f32_scaled(w.x, w.y, nl, rs[0], cs[0], sn[0]);
+        box_muller_f32_scaled(w.z, w.w, nl, rs[1], cs[1], sn[1]);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int y = 2 * p + h;
           const cpx<float> v = beta[y];
-          const float q = v.re * v.re + v.im * v.im - 1.0f;
-          const float cr = A - Ui * q, ci = Bc + Ur * q;  // dt (rsl + i U (|b|^2 - 1))
-          const float nr = v.re + (cr * v.re - ci * v.im) + fmaf(kfac, z[y].re, fr) + sig * (h ? xr1 : xr0);
-          const float ni = v.im + (cr * v.im + ci * v.re) + fmaf(kfac, z[y].im, fi) + sig * (h ? xi1 : xi0);
-          beta[y] = mk(nr, ni);
+          const float m = fmaf(v.im, v.im, v.re * v.re);
+          const float cr = fmaf(-Ui, m, A0), ci = fmaf(Ur, m, B0);
+          // increment = noise + dt iF + dt h + drift, all small, added to beta once
+          float dr = fmaf(rs[h], cs[h], fr);
+          dr = fmaf(kfac, z[y].re, dr);
+          dr = fmaf(cr, v.re, dr);
+          dr = fmaf(-ci, v.im, dr);
+          float di = fmaf(rs[h], sn[h], fi);
+          di = fmaf(kfac, z[y].im, di);
+          di = fmaf(cr, v.im, di);
+          di = fmaf(ci, v.re, di);
+          beta[y] = mk(v.re + dr, v.im + di);
         }
       }
     }
