@@ -113,3 +113,36 @@ def test_stacked_2d_items_equal_unstacked():
         # k-space bins are FP64 atomic sums over items: equal up to summation order
         np.testing.assert_allclose(got["ksum"][k], ref["ksum"][k], rtol=1e-12, atol=1e-12)
     sw.close()
+
+
+def test_sweep_full_length_error_vs_fp64():
+    """The sweep configuration's length (30000 steps at dt 0.002, frames every
+    500): three points of different shapes, 256 trajectories each, against
+    FP64 single runs (the LDS kernel, the reference's FFT form) of the same
+    trajectory ids and seed -- per-point <n> and g2(0) per frame in standard
+    errors of the FP64 ensemble."""
+    from pincosimulator_amd import observables
+    pts = [dict(nx=1, ny=64, n_traj=256, traj_id0=0, J=1.0, gamma=1.0, omega=-1.0, U=0.05, F=2.5),
+           dict(nx=16, ny=16, n_traj=256, traj_id0=1000, J=1.0, gamma=1.0, omega=1.0, U=0.05, F=2.0),
+           dict(nx=32, ny=32, n_traj=256, traj_id0=5000, J=1.0, gamma=1.0, omega=0.5, U=0.05, F=3.0)]
+    t_end = 60.0
+    sw = Sweep(pts, 0.002, 500, SEED)
+    sw.autotune(steps=20)
+    out = sw.run(t_end, graph=True)
+    sw.close()
+    for k, p in enumerate(pts):
+        s = System(nx=p["nx"], ny=p["ny"], dt=0.002, gamma=1.0, J=1.0, omega=p["omega"], U=0.05, F=p["F"],
+                   beta_init=0.0, beta_sigma=0.5, frame_steps=500, seed=SEED, t_start=0.0, t_end=t_end)
+        eng = Engine(s, precision=64, kernel="generic")
+        r = eng.run(np.arange(256) + p["traj_id0"])
+        eng.close()
+        m2, m4 = out["m2"][k], out["m4"][k]
+        assert m2.shape == r.traj_m2.shape == (256, 61)
+        n64, se_n = observables.mean_occupation(r)
+        g64, se_g = observables.g2_jackknife(r)
+        n32 = m2.mean(axis=0) - 0.5
+        g32 = (m4.mean(axis=0) - 2 * m2.mean(axis=0) + 0.5) / (m2.mean(axis=0) - 0.5) ** 2
+        dn = np.max(np.abs(n32 - n64)[1:] / se_n[1:])
+        dg = np.max(np.abs(g32 - g64)[1:] / se_g[1:])
+        print(f"point {k} ({p['nx']}x{p['ny']}): <n> {dn:.2e}, g2(0) {dg:.2e} standard errors over 60 frames")
+        assert dn < 3 and dg < 3
