@@ -117,6 +117,14 @@ def sweep_points():
     return pts
 
 
+def sweep_traffic():
+    """HBM bytes per step launch of the sweep kernel, from its counter profile."""
+    try:
+        return json.load(open(os.path.join(ROOT, "profiles", "sweep_512points_pmc.json")))["hbm_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def bench_sweep(args):
     """One step = one 500-step frame of all 512 points (the full run is 60)."""
     from pincosimulator_amd import Sweep
@@ -154,7 +162,7 @@ def bench_sweep(args):
                    "autotuned": not args.no_autotune,
                    "kernel_ms": out["kernel_ms"]},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None, "kernel": "sweep_step_kernel<false>",
+                     "frac": achieved / PEAK_FP32, "traffic": sweep_traffic(), "kernel": "sweep_step_kernel<false>",
                      "avg_launch_ms": out["kernel_ms"] / args.steps,
                      "flops_basis": "algorithmic: 10 log2 N + 30 flops per site-step of each point's lattice "
                                     "(the reference's FFT hopping), over the graph's kernel time per frame "

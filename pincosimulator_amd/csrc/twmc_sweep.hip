@@ -344,10 +344,13 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
         // k-space populations: forward FFT, rows (length ny) then columns
         // (length nx) for 2D points; |DFT beta|^2 per radial bin, summed over
         // this item's trajectories
-        cpx<float>* cur = fft_dim<-1>(bufA, bufB, P.py, nx, ny, 1, twy);
+        // the radix plans are read where they lie in global memory: indexing
+        // the local copy's plan arrays would put the whole copy in scratch
+        const SweepPointDev& Pg = g.points[item.point];
+        cpx<float>* cur = fft_dim<-1>(bufA, bufB, Pg.py, nx, ny, 1, twy);
         if (P.fft_x) {
           cpx<float>* other = (cur == bufA) ? bufB : bufA;
-          cur = fft_dim<-1>(cur, other, P.px, ny, 1, ny, twx);
+          cur = fft_dim<-1>(cur, other, Pg.px, ny, 1, ny, twx);
         }
         for (int i = threadIdx.x; i < m; i += blockDim.x) {
           const cpx<float> f = cur[i];
