@@ -53,3 +53,15 @@ def test_warm_start_and_sweep_entry_points_reject_bad_arguments_without_gpu():
     assert L.twmc_sweep_create(None, 0, 0.1, 1, 1, 0, C.byref(ctx)) == -1
     assert L.twmc_sweep_run(None, 0.0, 1.0, 0, None, None, None, None, None) == -1
     assert L.twmc_sweep_mode_bins(None, 0, None) == -1
+
+
+def test_kernel_ids_match_the_header():
+    """Every TWMC_KERNEL_* of the header has the same value and a name in the
+    Python bindings, and the Engine accepts each name."""
+    from pincosimulator_amd import engine
+    text = open(os.path.join(ROOT, "include", "twmc_hip.h")).read()
+    ids = {m.group(1).lower(): int(m.group(2)) for m in re.finditer(r"TWMC_KERNEL_(\w+)\s*=\s*(\d+)", text)}
+    assert ids and set(ids.values()) == set(_lib.KERNEL_NAMES)
+    for name, value in ids.items():
+        assert _lib.KERNEL_NAMES[value] == name
+        assert engine._KERNELS[name] == value
