@@ -84,3 +84,41 @@ def test_resume_continues_the_same_stream(tmp_path):
         assert rel_err(d["traj"].reshape(11, 16), fr) < 1e-9
         last = arc.load(tid, last_frame_only=True)
         assert np.array_equal(last["traj"][0], d["traj"][-1])
+
+
+SQUARE32 = """Manager = "TWMC"
+nx = {n}
+ny = {n}
+PBC = true
+J = 1.0
+omega = 1.0
+U = 0.05
+F = 2.0
+gamma = 1
+beta_init = 0
+beta_init_sigma = 0.5
+t_start = 0
+t_end = 0.4
+timestep = 0.002
+frames_freq = 5
+n_traj = 4
+SEED = 7
+precision = 32
+"""
+
+
+@pytest.mark.parametrize("n", [16, 64])
+def test_fp32_ini_runs_on_the_fast_kernels(tmp_path, n):
+    """precision = 32 in the .ini: the plugin's AUTO choice is the stacked
+    stencil (16x16) or the register-resident 64x64 kernel; the archive's
+    trajectories match the FP64 oracle to the FP32 bar."""
+    arc = run_sim(str(tmp_path / f"sq{n}"), SQUARE32.format(n=n))
+    assert arc.ids() == [1, 2, 3, 4]
+    p = dict(nx=n, ny=n, lattice=oracle.LAT_FFT, dt=0.002, gamma=1.0, J=1.0, omega=1.0, U=0.05, F=2.0,
+             beta_init=0.0, beta_sigma=0.5, frame_steps=100, seed=7, t_start=0.0, t_end=0.4)
+    for tid in arc.ids():
+        d = arc.load(tid)
+        fr, _, _ = oracle.run_trajectory(p, tid)
+        assert d["traj"].shape == (3, n, n)
+        err = rel_err(d["traj"].reshape(3, n * n), fr)
+        assert 1e-10 < err < 2e-4  # FP32 rounding is visible: the FP32 kernels ran
