@@ -181,7 +181,7 @@ int twmc_run(void* ctx, const twmc_batch_desc* d, twmc_output* out);
 int twmc_set_timing(void* ctx, int enable);   /* HIP events around step launches */
 int twmc_query(void* ctx, twmc_stats* st);
 int twmc_reset_stats(void* ctx);
-float twmc_progress(const void* ctx);         /* fraction of the batch's steps enqueued and done */
+float twmc_progress(const void* ctx);         /* fraction of the batch's steps enqueued; any thread may call it */
 
 /* Host-side NoisyMatrix realisation with the kernels' Philox layout (tag as
  * in twmc_rng.h).  Used for one shared disorder realisation. */

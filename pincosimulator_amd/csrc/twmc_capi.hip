@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -86,6 +87,8 @@ struct Ctx {
   std::vector<Segment> segs;
   size_t next_seg = 0;
   uint64_t steps_done = 0;
+  // fraction of the batch's steps enqueued, read by other threads (twmc_progress)
+  std::atomic<float> progress{0.f};
   int grid = 0, block = 256;
   int cap_grid = 0;  // stencil64: grid of the capture launch (the 64x64 FFT kernel's)
   size_t lds = 0;
@@ -158,10 +161,15 @@ std::vector<unsigned char> pack_complex(const std::vector<double>& v) {
   return out;
 }
 
+// Host-to-device copy that has finished when it returns: the sources are
+// temporaries of the caller (packed tables, the caller's trajectory ids and
+// previous states) that may be released as soon as the call returns.
 hipError_t upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
   hipError_t e = b.ensure(bytes);
   if (e != hipSuccess || bytes == 0) return e;
-  return hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s);
+  e = hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e;
 }
 
 // The reference's time loop (TWMCBaseSolver.cpp:221-274) run on the host to
@@ -663,6 +671,7 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
   plan_segments(c);
   c->next_seg = 0;
   c->steps_done = 0;
+  c->progress.store(0.f, std::memory_order_relaxed);
 
   const int n = c->n;
   const size_t B = d->n_traj;
@@ -915,6 +924,8 @@ int twmc_batch_advance(void* ctx, uint64_t n_segments) {
         }
       }
       c->steps_done += (uint64_t)steps;
+      if (c->n_steps > 0)
+        c->progress.store(float(double(c->steps_done) / double(c->n_steps)), std::memory_order_relaxed);
       left -= (uint64_t)steps;
       first = false;
     } while (left > 0);
@@ -1039,9 +1050,7 @@ int twmc_reset_stats(void* ctx) {
 
 float twmc_progress(const void* ctx) {
   if (!ctx) return 0.f;
-  const Ctx* c = static_cast<const Ctx*>(ctx);
-  if (!c->batch_active || c->n_steps == 0) return 0.f;
-  return float(double(c->steps_done) / double(c->n_steps));
+  return static_cast<const Ctx*>(ctx)->progress.load(std::memory_order_relaxed);
 }
 
 int twmc_noisy_matrix(const twmc_matrix* m, uint64_t n, uint64_t seed, uint64_t traj, uint32_t tag,
