@@ -53,7 +53,7 @@ __device__ __forceinline__ void split_bf16(float x0, float x1, uint32_t& hi, uin
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(d, bf16x2));
 }
 
-template <int TPW, bool CAPTURE_ONLY>
+template <int TPW, bool OVL, bool CAPTURE_ONLY>
 __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int np = g.np, n = g.n;
@@ -94,6 +94,12 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
     const bool valid = j < g.n_traj;
     const uint64_t traj = valid ? g.traj_ids[j] : 0;
     cpx<float>* gb = reinterpret_cast<cpx<float>*>(g.beta) + (size_t)(valid ? j : 0) * n;
+    // first site of this lane's tiles, opaque to the compiler: the per-site
+    // addresses and bounds of the load, capture and write-back are then
+    // formed in the block, not hoisted out of the block loop and held
+    // across the steps
+    int ib = wave * TPW * 32 + half * 4;
+    asm volatile("" : "+v"(ib));
 
     // FP32 state in the MFMA result layout: [tile][register]
     float sre[TPW][16], sim[TPW][16];
@@ -101,13 +107,71 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
     for (int t = 0; t < TPW; ++t)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int i = (wave * TPW + t) * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+        const int i = ib + t * 32 + (r >> 2) * 8 + (r & 3);
         cpx<float> v = (valid && wave * TPW + t < tiles && i < n) ? gb[i] : mk(0.f, 0.f);
         sre[t][r] = v.re;
         sim[t][r] = v.im;
       }
 
-    const int iters = CAPTURE_ONLY ? 1 : g.n_steps;
+    // frame capture of the state at the start of the segment, before the
+    // step loop (inside it, its per-site addresses would be held across the
+    // steps)
+    if (g.capture) {
+      double m2 = 0.0, m4 = 0.0;
+      cpx<float>* out = (g.dump && valid)
+                            ? reinterpret_cast<cpx<float>*>(g.traj_out) + ((size_t)j * g.n_frames + g.frame) * n
+                            : nullptr;
+#pragma unroll
+      for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int i = ib + t * 32 + (r >> 2) * 8 + (r & 3);
+          const bool use = valid && wave * TPW + t < tiles && i < n;
+          const double xr = use ? double(sre[t][r]) : 0.0, xi = use ? double(sim[t][r]) : 0.0;
+          const double nn = xr * xr + xi * xi;
+          m2 += nn;
+          m4 += nn * nn;
+          if (out && use) out[i] = mk(sre[t][r], sim[t][r]);
+          // per-site sums over the block's 32 trajectories: the 32 lanes of
+          // this half-wave hold the same site for different trajectories
+          double a = nn, b = nn * nn, c = xr, d = xi;
+#pragma unroll
+          for (int o = 1; o < 32; o <<= 1) {
+            a += __shfl_xor(a, o, 64);
+            b += __shfl_xor(b, o, 64);
+            c += __shfl_xor(c, o, 64);
+            d += __shfl_xor(d, o, 64);
+          }
+          if (col == 0 && wave * TPW + t < tiles && i < n) {
+            part[i] += a;
+            part[n + i] += b;
+            part[3 * n + i] += c;
+            part[4 * n + i] += d;
+          }
+        }
+      // lanes col and col+32 hold the same trajectory
+      m2 += __shfl_xor(m2, 32, 64);
+      m4 += __shfl_xor(m4, 32, 64);
+      if (half == 0) {
+        red[(wave * 32 + col) * 2] = m2;
+        red[(wave * 32 + col) * 2 + 1] = m4;
+      }
+      __syncthreads();
+      if (tid < 32 && blk * kTB + tid < g.n_traj) {
+        double a2 = 0.0, a4 = 0.0;
+        for (int w = 0; w < 4; ++w) {
+          a2 += red[(w * 32 + tid) * 2];
+          a4 += red[(w * 32 + tid) * 2 + 1];
+        }
+        const size_t o = (size_t)(blk * kTB + tid) * g.n_frames + g.frame;
+        g.traj_m2[o] = a2 / double(n);
+        g.traj_m4[o] = a4 / double(n);
+      }
+      __syncthreads();
+    }
+    if (CAPTURE_ONLY) continue;
+
+    const int iters = g.n_steps;
     int ft_id = 0;
     for (int s = 0; s < iters; ++s) {
       // stage the state as the BF16 hi/lo B operand: column col (Re) and 32+col (Im)
@@ -130,65 +194,11 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
         }
       __syncthreads();
 
-      if (s == 0 && g.capture) {
-        double m2 = 0.0, m4 = 0.0;
-        cpx<float>* out = (g.dump && valid)
-                              ? reinterpret_cast<cpx<float>*>(g.traj_out) + ((size_t)j * g.n_frames + g.frame) * n
-                              : nullptr;
-#pragma unroll
-        for (int t = 0; t < TPW; ++t)
-#pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const int i = (wave * TPW + t) * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
-            const bool use = valid && wave * TPW + t < tiles && i < n;
-            const double xr = use ? double(sre[t][r]) : 0.0, xi = use ? double(sim[t][r]) : 0.0;
-            const double nn = xr * xr + xi * xi;
-            m2 += nn;
-            m4 += nn * nn;
-            if (out && use) out[i] = mk(sre[t][r], sim[t][r]);
-            // per-site sums over the block's 32 trajectories: the 32 lanes of
-            // this half-wave hold the same site for different trajectories
-            double a = nn, b = nn * nn, c = xr, d = xi;
-#pragma unroll
-            for (int o = 1; o < 32; o <<= 1) {
-              a += __shfl_xor(a, o, 64);
-              b += __shfl_xor(b, o, 64);
-              c += __shfl_xor(c, o, 64);
-              d += __shfl_xor(d, o, 64);
-            }
-            if (col == 0 && wave * TPW + t < tiles && i < n) {
-              part[i] += a;
-              part[n + i] += b;
-              part[3 * n + i] += c;
-              part[4 * n + i] += d;
-            }
-          }
-        // lanes col and col+32 hold the same trajectory
-        m2 += __shfl_xor(m2, 32, 64);
-        m4 += __shfl_xor(m4, 32, 64);
-        if (half == 0) {
-          red[(wave * 32 + col) * 2] = m2;
-          red[(wave * 32 + col) * 2 + 1] = m4;
-        }
-        __syncthreads();
-        if (tid < 32 && blk * kTB + tid < g.n_traj) {
-          double a2 = 0.0, a4 = 0.0;
-          for (int w = 0; w < 4; ++w) {
-            a2 += red[(w * 32 + tid) * 2];
-            a4 += red[(w * 32 + tid) * 2 + 1];
-          }
-          const size_t o = (size_t)(blk * kTB + tid) * g.n_frames + g.frame;
-          g.traj_m2[o] = a2 / double(n);
-          g.traj_m4[o] = a4 / double(n);
-        }
-        __syncthreads();
-      }
-      if (CAPTURE_ONLY) break;
-
       if (g.ft_n > 0) {
         // F(t) before this step into the site table (TWMCLiebSolver.cpp:252-256);
-        // the epilogue reads it after the barrier that follows the GEMM, and
-        // the previous epilogue finished before the staging barrier above
+        // the epilogue reads it after the barrier that follows the GEMM (the
+        // overlapped step, which reads it during the GEMM, after one of its
+        // own), and the previous epilogue finished before the staging barrier
         const float fdt = float(drive_interval(g.ft_times, g.ft_n, g.t0 + double(s) * g.dt, ft_id));
         const cpx<float>* fv = reinterpret_cast<const cpx<float>*>(g.ft_vals) + (size_t)ft_id * n;
         const cpx<float>* fs = reinterpret_cast<const cpx<float>*>(g.ft_slopes) + (size_t)ft_id * n;
@@ -196,6 +206,139 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
           PF[i] = mk(-dt * fmaf(fdt, fs[i].im, fv[i].im), dt * fmaf(fdt, fs[i].re, fv[i].re));
       }
 
+      if constexpr (OVL) {
+        // np = 128 TPW: 8 TPW k blocks and 8 TPW site pairs per lane.  The
+        // part of the epilogue that does not depend on Y = C^T beta -- the
+        // pair's Philox draw, Box-Muller and the drift and drive terms of the
+        // increment -- runs during the GEMM, one site pair per k block, its
+        // pieces placed between the block's MFMAs (fenced, so the order is
+        // kept and the matrix cores run under the VALU work); after the GEMM
+        // only dt h is added.
+        constexpr int NKB = 8 * TPW, kPieces = 9, kSlots = 6 * TPW;
+        if (g.ft_n > 0) __syncthreads();  // the F(t) table written above is read during the GEMM
+        const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
+        // opaque to the compiler: the per-pair Philox rounds, the table reads
+        // and the A addresses are formed in the step, not hoisted out of it
+        uint32_t pbase = (uint32_t)(wave * TPW * 16 + half * 2);
+        asm volatile("" : "+v"(pbase));
+        int zo = 0;
+        asm volatile("" : "+s"(zo));
+        const size_t abase = ((size_t)(wave * TPW) * NKB * 64 + lane) * 8 + zo;
+        auto aload = [&](const uint16_t* A, int t, int kb) {
+          return *reinterpret_cast<const bf16x8*>(&A[abase + ((size_t)t * NKB + kb) * 512]);
+        };
+        auto bload = [&](const uint16_t* X, int c, int kb) {
+          return *reinterpret_cast<const bf16x8*>(&X[c * ldk + kb * 16 + half * 8 + zo]);
+        };
+        float pre_r[TPW][16], pre_i[TPW][16];
+        f32x16 are[TPW], aim[TPW];
+#pragma unroll
+        for (int t = 0; t < TPW; ++t)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            are[t][r] = 0.f;
+            aim[t][r] = 0.f;
+          }
+        bf16x8 ah[TPW], al[TPW], bq[4];
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+          ah[t] = aload(Ahi, t, 0);
+          al[t] = aload(Alo, t, 0);
+        }
+        bq[0] = bload(Xhi, col, 0);
+        bq[1] = bload(Xhi, 32 + col, 0);
+        bq[2] = bload(Xlo, col, 0);
+        bq[3] = bload(Xlo, 32 + col, 0);
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb) {
+          // operands of the next k block, loaded one block ahead
+          bf16x8 ahn[TPW], aln[TPW], bn[4];
+          if (kb + 1 < NKB) {
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+              ahn[t] = aload(Ahi, t, kb + 1);
+              aln[t] = aload(Alo, t, kb + 1);
+            }
+            bn[0] = bload(Xhi, col, kb + 1);
+            bn[1] = bload(Xhi, 32 + col, kb + 1);
+            bn[2] = bload(Xlo, col, kb + 1);
+            bn[3] = bload(Xlo, 32 + col, kb + 1);
+          }
+          // site pair kb: row tile tp, result registers 4 q + 2 pr + 0/1
+          const int tp = kb >> 3, q = (kb >> 1) & 3, pr = kb & 1;
+          u32x4 w;
+          float rn[2], cs[2], sn[2];
+          auto piece = [&](int pc) __attribute__((always_inline)) {
+            if (pc == 0) w = make_counter(traj, TAG_WIENER, step, pbase + (uint32_t)(tp * 16 + q * 4 + pr));
+            // each piece's results pass through an empty volatile asm, which
+            // the fences order: pure arithmetic alone could be moved across
+            // them
+            if (pc < 5) {
+              philox_round(w, g.key0, g.key1, 2 * pc);
+              philox_round(w, g.key0, g.key1, 2 * pc + 1);
+              asm volatile("" : "+v"(w.x), "+v"(w.y), "+v"(w.z), "+v"(w.w));
+            } else if (pc < 7) {
+              const int h = pc - 5;
+              box_muller_f32_scaled(h ? w.z : w.x, h ? w.w : w.y, nl, rn[h], cs[h], sn[h]);
+              asm volatile("" : "+v"(rn[h]), "+v"(cs[h]), "+v"(sn[h]));
+            } else {
+              // increment without dt h: noise + dt iF + drift (TWMCLiebSolver.cpp:259-264)
+              const int h = pc - 7, r = 4 * q + 2 * pr + h;
+              const int i = (wave * TPW + tp) * 32 + q * 8 + half * 4 + 2 * pr + h;
+              const float br = sre[tp][r], bi = sim[tp][r];
+              const cpx<float> ab = Prsl[i + zo], u = PU[i + zo], f = PF[i + zo];
+              const float mq = fmaf(bi, bi, br * br);
+              const float cr = fmaf(-u.im, mq, ab.re), ci = fmaf(u.re, mq, ab.im);
+              float dr = fmaf(rn[h], cs[h], f.re);
+              dr = fmaf(cr, br, dr);
+              pre_r[tp][r] = fmaf(-ci, bi, dr);
+              float di = fmaf(rn[h], sn[h], f.im);
+              di = fmaf(cr, bi, di);
+              pre_i[tp][r] = fmaf(ci, br, di);
+              asm volatile("" : "+v"(pre_r[tp][r]), "+v"(pre_i[tp][r]));
+            }
+          };
+#pragma unroll
+          for (int sl = 0; sl < kSlots; ++sl) {
+#pragma unroll
+            for (int pc = (sl * kPieces + kSlots - 1) / kSlots; pc < ((sl + 1) * kPieces + kSlots - 1) / kSlots; ++pc)
+              piece(pc);
+            // MFMA sl: product wm = sl / TPW of row tile tt = sl % TPW, in the
+            // order hi.hi Re, hi.hi Im, hi.lo Re, hi.lo Im, lo.hi Re, lo.hi Im
+            // (consecutive MFMAs on different accumulators)
+            const int wm = sl / TPW, tt = sl % TPW;
+            const bf16x8 a = wm < 4 ? ah[tt] : al[tt];
+            const bf16x8 b = bq[wm < 4 ? wm : wm - 4];
+            if (wm & 1)
+              aim[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, aim[tt], 0, 0, 0);
+            else
+              are[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, are[tt], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+          if (kb + 1 < NKB) {
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+              ah[t] = ahn[t];
+              al[t] = aln[t];
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) bq[e] = bn[e];
+          }
+        }
+        __syncthreads();  // the next step overwrites the B operand
+        // dt h, h = -i Y (h_re = Im Y, h_im = -Re Y), and the update
+#pragma unroll
+        for (int t = 0; t < TPW; ++t)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int i = (wave * TPW + t) * 32 + (r >> 2) * 8 + half * 4 + (r & 3) + zo;
+            if (i < n) {
+              sre[t][r] += fmaf(dt, aim[t][r], pre_r[t][r]);
+              sim[t][r] += fmaf(-dt, are[t][r], pre_i[t][r]);
+            }
+          }
+        continue;
+      }
       f32x16 are[TPW], aim[TPW];
 #pragma unroll
       for (int t = 0; t < TPW; ++t) {
@@ -318,23 +461,32 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
       for (int t = 0; t < TPW; ++t)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const int i = (wave * TPW + t) * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+          const int i = ib + t * 32 + (r >> 2) * 8 + (r & 3);
           if (wave * TPW + t < tiles && i < n) gb[i] = mk(sre[t][r], sim[t][r]);
         }
     }
   }
 }
 
-template <int TPW>
+template <int TPW, bool OVL>
 hipError_t launch_t(const StepArgs& a, int grid, size_t lds, hipStream_t s) {
-  const void* f = a.n_steps > 0 ? (const void*)mfma_step_kernel<TPW, false> : (const void*)mfma_step_kernel<TPW, true>;
+  const void* f = a.n_steps > 0 ? (const void*)mfma_step_kernel<TPW, OVL, false>
+                                : (const void*)mfma_step_kernel<TPW, false, true>;
   hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   if (a.n_steps > 0)
-    hipLaunchKernelGGL((mfma_step_kernel<TPW, false>), dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((mfma_step_kernel<TPW, OVL, false>), dim3(grid), dim3(256), lds, s, a);
   else
-    hipLaunchKernelGGL((mfma_step_kernel<TPW, true>), dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((mfma_step_kernel<TPW, false, true>), dim3(grid), dim3(256), lds, s, a);
   return hipGetLastError();
+}
+
+// row tiles per wave, and whether the step overlaps the epilogue with the
+// GEMM: that needs one k block per site pair of a lane, np = 128 TPW (the
+// 384-site Lieb chain is np = 384, TPW = 3), and TPW <= 3 for the registers
+void mfma_shape(int np, int* tpw, bool* ovl) {
+  *tpw = (np / 32 + 3) / 4;
+  *ovl = *tpw <= 3 && np == 128 * *tpw;
 }
 
 }  // namespace
@@ -365,24 +517,27 @@ bool mfma_supported(int n, int hopping, int precision, int param_stride) {
 }
 
 hipError_t launch_mfma(const StepArgs& a, int grid, hipStream_t s) {
-  const int tiles = a.np / 32;
-  const int tpw = (tiles + 3) / 4;
+  int tpw;
+  bool ovl;
+  mfma_shape(a.np, &tpw, &ovl);
   const size_t lds = mfma_lds_bytes(a.np);
   switch (tpw) {
-    case 1: return launch_t<1>(a, grid, lds, s);
-    case 2: return launch_t<2>(a, grid, lds, s);
-    case 3: return launch_t<3>(a, grid, lds, s);
-    default: return launch_t<4>(a, grid, lds, s);
+    case 1: return ovl ? launch_t<1, true>(a, grid, lds, s) : launch_t<1, false>(a, grid, lds, s);
+    case 2: return ovl ? launch_t<2, true>(a, grid, lds, s) : launch_t<2, false>(a, grid, lds, s);
+    case 3: return ovl ? launch_t<3, true>(a, grid, lds, s) : launch_t<3, false>(a, grid, lds, s);
+    default: return launch_t<4, false>(a, grid, lds, s);
   }
 }
 
 hipError_t mfma_occupancy(int np, int* blocks_per_cu) {
   const size_t lds = mfma_lds_bytes(np);
-  const int tpw = (np / 32 + 3) / 4;
-  const void* f = tpw == 1 ? (const void*)mfma_step_kernel<1, false>
-                  : tpw == 2 ? (const void*)mfma_step_kernel<2, false>
-                  : tpw == 3 ? (const void*)mfma_step_kernel<3, false>
-                             : (const void*)mfma_step_kernel<4, false>;
+  int tpw;
+  bool ovl;
+  mfma_shape(np, &tpw, &ovl);
+  const void* f = tpw == 1   ? (ovl ? (const void*)mfma_step_kernel<1, true, false> : (const void*)mfma_step_kernel<1, false, false>)
+                  : tpw == 2 ? (ovl ? (const void*)mfma_step_kernel<2, true, false> : (const void*)mfma_step_kernel<2, false, false>)
+                  : tpw == 3 ? (ovl ? (const void*)mfma_step_kernel<3, true, false> : (const void*)mfma_step_kernel<3, false, false>)
+                             : (const void*)mfma_step_kernel<4, false, false>;
   hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 256, lds);

@@ -76,6 +76,22 @@ __host__ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, ui
   return c;
 }
 
+// Round r (0-based) of philox4x32_10 alone, with that round's key
+// (k0 + r 0x9E3779B9, k1 + r 0xBB67AE85), for callers that spread the ten
+// rounds over other work; rounds 0..9 in order give philox4x32_10 exactly.
+__host__ __device__ __forceinline__ void philox_round(u32x4& c, uint32_t k0, uint32_t k1, int r) {
+  const uint32_t kr0 = k0 + uint32_t(r) * 0x9E3779B9u, kr1 = k1 + uint32_t(r) * 0xBB67AE85u;
+  uint32_t hi0, lo0, hi1, lo1;
+  mulhilo32(0xD2511F53u, c.x, hi0, lo0);
+  mulhilo32(0xCD9E8D57u, c.z, hi1, lo1);
+  u32x4 n;
+  n.x = r < 2 ? (hi1 ^ c.y ^ kr0) : xor3(hi1, c.y, kr0);
+  n.y = lo1;
+  n.z = r < 2 ? (hi0 ^ c.w ^ kr1) : xor3(hi0, c.w, kr1);
+  n.w = lo0;
+  c = n;
+}
+
 __host__ __device__ __forceinline__ u32x4 make_counter(uint64_t traj, uint32_t tag, uint32_t step, uint32_t pair) {
   u32x4 c;
   c.x = step;
