@@ -75,6 +75,18 @@ def test_drive_table_mfma_matches_oracle():
     assert rel_err(res.traj[:8], ref) < 2e-4
 
 
+def test_drive_table_mfma_overlapped_step_matches_oracle():
+    """384 sites: the MFMA step that reads the F(t) table during the GEMM."""
+    spec = lieb_spec(128, t_end=0.6, frame_steps=40)
+    ids = np.array([5, 17, 40])
+    print("F(t) fp32 mfma, 384 sites", flush=True)
+    eng = Engine(system(spec), precision=32, kernel="mfma")
+    res = eng.run(ids, traj=True)
+    eng.close()
+    assert res.kernel == "mfma"
+    assert rel_err(res.traj, oracle_frames(spec, ids)) < 2e-4
+
+
 def test_drive_table_is_not_sent_to_the_fused_kernel():
     spec = dict(nx=64, ny=64, dt=0.002, gamma=1.0, J=1.0, omega=1.0, U=0.05, F=0.0, beta_init=0.0,
                 beta_sigma=0.5, frame_steps=10, seed=1, t_start=0.0, t_end=0.02,

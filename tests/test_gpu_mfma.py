@@ -108,3 +108,26 @@ def test_mfma_ensemble_sums_match_dump():
     for k in ("sum_n", "sum_n2", "sum_beta", "traj_m2", "traj_m4"):
         assert rel_err(getattr(res, k), ref[k]) < 1e-9, k
     eng.close()
+
+
+@pytest.mark.parametrize("n", [128, 250, 380])
+def test_overlapped_step_sizes_vs_fp64_oracle(n):
+    """Dense random couplings at the sizes whose step runs the noise and drift
+    of one site pair per k block between the GEMM's MFMAs (padded sizes 128,
+    256 and 384: one, two and three row tiles per wave), 250 and 380 with
+    padded sites."""
+    rng = np.random.default_rng(n)
+    C = rng.uniform(-0.05, 0.05, (n, n))
+    C = C + C.T
+    spec = dict(nx=1, ny=n, dt=0.005, gamma=1.0, J=0.0, U=0.1, F=1.2, omega=0.2 * rng.standard_normal(n),
+                beta_init=0.0, beta_sigma=0.5, frame_steps=50, seed=91, t_start=0.0, t_end=0.5, lattice="matrix",
+                coupling=C, detuning=0.3, onsite="detuning-E")
+    ids = np.array([0, 7, 33])  # two 32-trajectory blocks
+    print(f"mfma {n} sites", flush=True)
+    eng = Engine(System(**spec), precision=32, kernel="mfma")
+    res = eng.run(ids, traj=True)
+    eng.close()
+    err = max(rel_err(res.traj[j], oracle.run_trajectory(dict(spec, lattice=oracle.LAT_DENSE, E=spec["omega"]),
+                                                         int(t))[0]) for j, t in enumerate(ids))
+    print(f"  MFMA {n} sites vs FP64 oracle, 100 steps: {err:.2e}")
+    assert err < 1e-3
