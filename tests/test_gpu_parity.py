@@ -3,12 +3,16 @@
 Bar: every stored frame of every trajectory within 1e-9 relative (max |diff| /
 max |beta|) after 1000 steps; frame counts, step counts and t_final identical.
 """
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 
 import oracle
-from helpers import c1_spec, oracle_traj, rel_err, system
-from pincosimulator_amd import Engine
+from helpers import c1_spec, oracle_params, oracle_traj, rel_err, system
+from pincosimulator_amd import Engine, observables
+from pincosimulator_amd.observables import from_trajectories
 
 pytestmark = pytest.mark.gpu
 
@@ -49,6 +53,41 @@ def test_chain64_fp64_1000_steps():
     assert rel_err(res.traj, frames) < TOL64
     assert rel_err(res.beta_final, finals) < TOL64
     assert res.t_final == tfs[0]
+
+
+def test_chain64_configuration_in_full_fp64():
+    """The 1D chain configuration as specified -- N=64 PBC, 256 trajectories,
+    20000 steps of dt = 0.005, 101 frames, complex128 -- on the FP64 GPU path
+    against the FP64 oracle run on every trajectory (a thread pool of oracle
+    calls): per-trajectory deviation over all frames, and the GPU's FP64
+    ensemble sums and observables against the same estimators built from the
+    oracle's frames."""
+    spec = c1_spec()
+    ids = np.arange(256)
+    print("chain64 in full: FP64, 256 trajectories x 20000 steps", flush=True)
+    eng = Engine(system(spec), precision=64)
+    res = eng.run(ids, traj=True, final=True)
+    eng.close()
+    assert res.kernel == "generic" and res.n_steps == 20000 and res.n_frames == 101
+    p = oracle_params(spec)
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        out = list(ex.map(lambda t: oracle.run_trajectory(p, int(t)), ids))
+    frames = np.array([o[0] for o in out])
+    dev = rel_err(res.traj, frames)
+    dev_final = rel_err(res.beta_final, np.array([o[1] for o in out]))
+    ref = from_trajectories(frames, (1, 64))
+    dsum = {k: rel_err(getattr(res, k), ref[k]) for k in ("sum_n", "sum_n2", "sum_nk", "sum_beta")}
+    n_gpu, _ = observables.mean_occupation(res)
+    g_gpu, g_se = observables.g2_jackknife(res)
+    a2 = np.abs(frames) ** 2
+    n_ora = a2.mean(axis=(0, 2)) - 0.5
+    print(f"  per trajectory, all frames: {dev:.2e}; final states {dev_final:.2e}")
+    print("  ensemble sums: " + ", ".join(f"{k} {v:.2e}" for k, v in dsum.items()))
+    print(f"  <n> per frame: max |GPU - oracle| = {np.max(np.abs(n_gpu - n_ora)):.2e}; "
+          f"g2(0) at the last frame {g_gpu[-1]:.4f} +- {g_se[-1]:.4f}")
+    assert dev < TOL64 and dev_final < TOL64
+    assert all(v < TOL64 for v in dsum.values())
+    assert np.max(np.abs(n_gpu - n_ora)) < TOL64
 
 
 @pytest.mark.parametrize("nx,ny", [(8, 8), (12, 10), (48, 48), (16, 15), (2, 2)])
