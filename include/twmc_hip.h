@@ -218,7 +218,9 @@ int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out);
  * n_frames x 34) the FP64 sums over trajectories of |DFT(beta)_k|^2 per radial
  * bin (32 bins, see twmc_sweep_mode_bins) and of sum_i beta_i (Re, Im);
  * use_graph captures the frame loop into one hipGraph; kernel_ms receives the
- * HIP-event time.  Any output pointer may be NULL. */
+ * HIP-event time.  Any output pointer may be NULL.  The run stores at most
+ * nearbyint((t_end - t_start) / (dt frame_steps)) + 1 frames (n_frames_out
+ * the actual count): size the outputs for that many. */
 int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, double* traj_m2, double* traj_m4,
                    double* ksum, uint64_t* n_frames_out, double* kernel_ms);
 /* radial bin (0..31) of every mode of point `point` (nx * ny entries, row-major):

@@ -83,6 +83,8 @@ def _as_matrix(spec, n, keep):
             m.amp_value[0], m.amp_value[1] = a.real, a.imag
         else:
             arr = np.ascontiguousarray(np.asarray(amp, dtype=np.complex128).reshape(-1))
+            if arr.size != n:
+                raise ValueError(f"site-resolved noise amplitude has {arr.size} entries, lattice has {n}")
             keep.append(arr)
             m.amp = arr.ctypes.data_as(C.POINTER(C.c_double))
     return m
@@ -220,6 +222,8 @@ class Engine:
         d.store_initial = int(bool(store_initial))
         if beta_prev is not None:
             bp = np.ascontiguousarray(np.asarray(beta_prev, dtype=np.complex128).reshape(ids.size, -1))
+            if bp.shape[1] != s.n_sites:
+                raise ValueError(f"beta_prev has {bp.shape[1]} sites per trajectory, lattice has {s.n_sites}")
             self._bp = bp
             d.beta_prev = bp.ctypes.data_as(C.POINTER(C.c_double))
         else:

@@ -67,3 +67,17 @@ def test_jackknife_matches_bootstrap_scale():
         a, b = m2[i].mean(), m4[i].mean()
         boots.append((b - 2 * a + 0.5) / (a - 0.5) ** 2)
     assert 0.7 < se[0] / np.std(boots) < 1.4
+
+
+def test_site_resolved_parameters_must_match_the_lattice():
+    """Site-resolved values and noise amplitudes are checked against the
+    lattice size before a pointer to them reaches the C ABI."""
+    import pytest
+    from pincosimulator_amd.engine import _as_matrix
+    keep = []
+    m = _as_matrix({"base": np.ones(6), "noise": ("gaussian", np.full(6, 0.1))}, 6, keep)
+    assert m.noise_type != 0 and len(keep) == 2
+    with pytest.raises(ValueError, match="lattice has 6"):
+        _as_matrix(np.ones(5), 6, [])
+    with pytest.raises(ValueError, match="noise amplitude has 5 entries"):
+        _as_matrix({"base": 1.0, "noise": ("uniform", np.ones(5))}, 6, [])
