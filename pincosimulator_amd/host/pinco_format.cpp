@@ -1,5 +1,6 @@
 #include "pinco_format.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
@@ -71,7 +72,7 @@ bool PincoFormatDataStore::ReadRegister() {
   dims_.assign(nd, {});
   for (int i = 0; i < nd; ++i) {
     size_t D = 0;
-    if (!rd(reg_, D)) return false;
+    if (!rd(reg_, D) || D > 64) return false;  // a corrupt count would allocate without bound
     dims_[i].resize(D);
     for (size_t k = 0; k < D; ++k)
       if (!rd(reg_, dims_[i][k])) return false;
@@ -212,9 +213,14 @@ std::unique_ptr<TaskResults> PincoFormatDataStore::LoadResult(size_t id, bool la
     for (size_t i = 0; i < names_.size(); ++i) {
       FILE* f = fopen(cnk_name(folder_, i, e->chunk).c_str(), "rb");
       if (!f) throw std::runtime_error("chunk file " + cnk_name(folder_, i, e->chunk) + " is missing");
+      fseek(f, 0, SEEK_END);
+      const size_t fsize = (size_t)ftell(f);
       fseek(f, (long)rec[2 + 2 * i], SEEK_SET);
       size_t nbytes = 0;
-      rd(f, nbytes);
+      if (!rd(f, nbytes) || nbytes > fsize - std::min(fsize, rec[2 + 2 * i] + sizeof(size_t))) {
+        fclose(f);
+        throw std::runtime_error("corrupt entry in chunk file " + cnk_name(folder_, i, e->chunk));
+      }
       size_t nfr = rec[1 + 2 * i];
       std::vector<unsigned char> buf(nbytes);
       if (nbytes && fread(buf.data(), 1, nbytes, f) != nbytes) {
