@@ -181,7 +181,7 @@ __device__ __forceinline__ void step_item(const SweepArgs& g, const SweepPointDe
 }
 
 template <bool CAPTURE_ONLY>
-__global__ void __launch_bounds__(kSweepBlock) sweep_step_kernel(SweepArgs g) {
+__global__ void __launch_bounds__(kSweepBlock, 3) sweep_step_kernel(SweepArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* red = reinterpret_cast<double*>(smem);  // [8]
   cpx<float>* bufA = reinterpret_cast<cpx<float>*>(smem + 64);
