@@ -29,8 +29,20 @@ __device__ __forceinline__ float from_next_lane(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x134, 0xf, 0xf, true));
 }
 
-template <int S, int W, bool FT>
+// the Lieb chain's bonds as offsets d of the coupled site j + d, by the site
+// index mod 3 (A_i = 3i, B_i = 3i + 1, C_i = 3i + 2; TWMCLiebSolver.cpp:52-85):
+// A-B (+1); B-A (-1), B-C (+1), B-C of the previous cell (-2); C-B (-1),
+// C-B of the next cell (+2)
+__host__ __device__ constexpr bool lieb_bond(int r, int d) {
+  return r == 0 ? d == 1 : r == 1 ? (d == -1 || d == 1 || d == -2) : (d == -1 || d == 2);
+}
+
+// LIEB (S = 6, W = 2): each lane's six sites are two whole cells, so the
+// offsets without a bond are known per register and their multiplications
+// by zero are left out (6 of the 10 complex products per cell remain)
+template <int S, int W, bool FT, bool LIEB>
 __global__ void __launch_bounds__(64) band_step_kernel(StepArgs g) {
+  static_assert(!LIEB || (S == 6 && W == 2), "the Lieb pattern needs two cells per lane");
   constexpr int D = 2 * W + 1;
   const int lane = threadIdx.x;
   const int n = g.n;
@@ -138,6 +150,7 @@ __global__ void __launch_bounds__(64) band_step_kernel(StepArgs g) {
       for (int s = 0; s < S; ++s)
 #pragma unroll
         for (int d = -W; d <= W; ++d) {
+          if (LIEB && !lieb_bond(s % 3, d)) continue;
           const int t = s + d;
           const float vre = t < 0 ? lre[W + t] : t >= S ? rre[t - S] : br[t];
           const float vim = t < 0 ? lim[W + t] : t >= S ? rim[t - S] : bi[t];
@@ -180,10 +193,19 @@ __global__ void __launch_bounds__(64) band_step_kernel(StepArgs g) {
 
 template <int S, int W>
 hipError_t launch_sw(const StepArgs& a, int grid, hipStream_t s) {
+  if constexpr (S == 6 && W == 2) {
+    if (a.band_lieb) {
+      if (a.ft_n > 0)
+        hipLaunchKernelGGL((band_step_kernel<6, 2, true, true>), dim3(grid), dim3(64), 0, s, a);
+      else
+        hipLaunchKernelGGL((band_step_kernel<6, 2, false, true>), dim3(grid), dim3(64), 0, s, a);
+      return hipGetLastError();
+    }
+  }
   if (a.ft_n > 0)
-    hipLaunchKernelGGL((band_step_kernel<S, W, true>), dim3(grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((band_step_kernel<S, W, true, false>), dim3(grid), dim3(64), 0, s, a);
   else
-    hipLaunchKernelGGL((band_step_kernel<S, W, false>), dim3(grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((band_step_kernel<S, W, false, false>), dim3(grid), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
@@ -235,6 +257,15 @@ void band_coefficients(const double* coupling, int n, int w, double dt, float* o
     }
 }
 
+bool band_lieb_pattern(const float* coef, int n, int w) {
+  const int S = band_sites_per_lane(n);
+  if (S != 6 || w != 2) return false;
+  for (int j = 0; j < 64 * S; ++j)
+    for (int d = -w; d <= w; ++d)
+      if (!lieb_bond(j % 3, d) && coef[(size_t)j * (2 * w + 1) + d + w] != 0.f) return false;
+  return true;
+}
+
 hipError_t launch_band(const StepArgs& a, int grid, hipStream_t s) {
   switch (band_sites_per_lane(a.n)) {
     case 2: return launch_s<2>(a, grid, s);
@@ -249,10 +280,10 @@ hipError_t band_occupancy(int n, int w, int* blocks_per_cu) {
   const int S = band_sites_per_lane(n);
   // the largest instantiation for this S bounds the registers of all of them
   switch (S) {
-    case 2: f = (const void*)band_step_kernel<2, 3, false>; break;
-    case 4: f = (const void*)band_step_kernel<4, 3, false>; break;
-    case 6: f = (const void*)band_step_kernel<6, 3, false>; break;
-    default: f = (const void*)band_step_kernel<8, 3, false>; break;
+    case 2: f = (const void*)band_step_kernel<2, 3, false, false>; break;
+    case 4: f = (const void*)band_step_kernel<4, 3, false, false>; break;
+    case 6: f = (const void*)band_step_kernel<6, 3, false, false>; break;
+    default: f = (const void*)band_step_kernel<8, 3, false, false>; break;
   }
   (void)w;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 64, 0);

@@ -73,6 +73,7 @@ struct Ctx {
   int ft_n = 0;  // F(t) table times (0: static F)
   int np = 0;  // padded site count of the MFMA kernel
   int band_w = -1;  // half-width of the coupling matrix (matrix hopping), -1: not banded
+  int band_lieb = 0;  // the banded coupling has the Lieb chain's pattern
   StackPlan stack;  // TWMC_KERNEL_STACK: the batch's point and work items on the sweep kernel
   // per-batch buffers
   DevBuf d_ids, d_beta, d_rsl, d_U, d_F, d_partials, d_obs, d_m2, d_m4, d_traj, d_prev;
@@ -593,6 +594,7 @@ int twmc_create(const twmc_params* p, int device, void** ctx_out) {
     const int S = band_sites_per_lane(c->n);
     std::vector<float> coef((size_t)64 * S * (2 * c->band_w + 1));
     band_coefficients(p->coupling, c->n, c->band_w, p->dt, coef.data());
+    c->band_lieb = band_lieb_pattern(coef.data(), c->n, c->band_w) ? 1 : 0;
     hipError_t ee = upload(c->d_band, coef.data(), coef.size() * sizeof(float), s);
     if (ee == hipSuccess) ee = hipStreamSynchronize(s);
     if (ee != hipSuccess) fail_rc = hip_fail(c, ee, "upload band coefficients");
@@ -879,6 +881,7 @@ int twmc_batch_advance(void* ctx, uint64_t n_segments) {
       if (c->noise[1] != TWMC_NOISE_NONE) a.u_real = 0;
       a.band_coef = static_cast<const float*>(c->d_band.p);
       a.band_w = c->band_w;
+      a.band_lieb = c->band_lieb;
       if (c->kernel == TWMC_KERNEL_STENCIL64) {
         int rc = advance_stencil(c, a, s);
         if (rc != TWMC_OK) return rc;

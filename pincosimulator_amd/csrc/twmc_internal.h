@@ -86,6 +86,7 @@ struct StepArgs {
   int u_real;                // 1: Im U = 0 at every site, in every trajectory
   const float* band_coef;    // banded kernel: [64 S][2W + 1] dt C_(j+d) j
   int band_w;                // banded kernel: half-width W
+  int band_lieb;             // banded kernel: the coupling has the Lieb chain's pattern (S = 6, W = 2)
   // time-dependent drive: F(t) = ft_vals[id] + (t - ft_times[id]) ft_slopes[id]
   // with id the last table time below t (F = ft_vals[0] up to ft_times[0]);
   // t = t0 + s dt before step s of this launch, in FP64
@@ -213,6 +214,9 @@ int band_sites_per_lane(int n);
 int band_width(const double* coupling, int n);  // W, or -1 when n is out of range
 bool band_supported(int n, int hopping, int precision, int param_stride, int width);
 void band_coefficients(const double* coupling, int n, int w, double dt, float* out);  // [64 S][2w+1]
+// true when the [64 S][2w+1] table has nonzeros only where a Lieb chain has
+// bonds (A: +1; B: -2, -1, +1; C: -1, +2, by site index mod 3) and S = 6, W = 2
+bool band_lieb_pattern(const float* coef, int n, int w);
 hipError_t launch_band(const StepArgs& a, int grid, hipStream_t s);
 hipError_t band_occupancy(int n, int w, int* blocks_per_cu);
 hipError_t mfma_occupancy(int np, int* blocks_per_cu);

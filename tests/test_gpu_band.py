@@ -67,7 +67,7 @@ def test_lieb_vs_fp64_oracle(cells, pbc):
 
 
 @pytest.mark.parametrize("n,w,cyclic", [(200, 3, False), (256, 2, True), (128, 1, True), (20, 2, False),
-                                         (512, 3, True)])
+                                         (512, 3, True), (384, 2, True), (330, 2, False)])
 def test_asymmetric_band_vs_fp64_oracle(n, w, cyclic):
     C = band_matrix(n, w, cyclic, seed=n + w)
     assert not np.allclose(C, C.T)
@@ -78,8 +78,10 @@ def test_asymmetric_band_vs_fp64_oracle(n, w, cyclic):
     assert err < 2e-4
 
 
-def test_time_dependent_drive():
-    cells = 16
+@pytest.mark.parametrize("cells", [16, 128])
+def test_time_dependent_drive(cells):
+    """F(t) on the banded kernel; 128 cells (384 sites, two cells per lane)
+    takes the variant that leaves out the Lieb chain's structural zeros."""
     rng = np.random.default_rng(5)
     times = np.array([0.1, 0.45, 0.8])
     vals = rng.normal(size=(3, 3 * cells)) + 1j * rng.normal(size=(3, 3 * cells))
