@@ -275,17 +275,28 @@ hipError_t launch_band(const StepArgs& a, int grid, hipStream_t s) {
   }
 }
 
-hipError_t band_occupancy(int n, int w, int* blocks_per_cu) {
-  const void* f = nullptr;
-  const int S = band_sites_per_lane(n);
-  // the largest instantiation for this S bounds the registers of all of them
-  switch (S) {
-    case 2: f = (const void*)band_step_kernel<2, 3, false, false>; break;
-    case 4: f = (const void*)band_step_kernel<4, 3, false, false>; break;
-    case 6: f = (const void*)band_step_kernel<6, 3, false, false>; break;
-    default: f = (const void*)band_step_kernel<8, 3, false, false>; break;
+// occupancy of the instantiation launch_band runs for (n, w, lieb) (the
+// FT = false variant, the larger of the two)
+template <int S>
+const void* band_kernel_for(int w, int lieb) {
+  if constexpr (S == 6) {
+    if (lieb && w == 2) return (const void*)band_step_kernel<6, 2, false, true>;
   }
-  (void)w;
+  switch (w) {
+    case 1: return (const void*)band_step_kernel<S, 1, false, false>;
+    case 2: return (const void*)band_step_kernel<S, 2, false, false>;
+    default: return (const void*)band_step_kernel<S, 3, false, false>;
+  }
+}
+
+hipError_t band_occupancy(int n, int w, int lieb, int* blocks_per_cu) {
+  const void* f = nullptr;
+  switch (band_sites_per_lane(n)) {
+    case 2: f = band_kernel_for<2>(w, lieb); break;
+    case 4: f = band_kernel_for<4>(w, lieb); break;
+    case 6: f = band_kernel_for<6>(w, lieb); break;
+    default: f = band_kernel_for<8>(w, lieb); break;
+  }
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 64, 0);
 }
 

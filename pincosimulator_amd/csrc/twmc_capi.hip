@@ -704,7 +704,7 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
   } else if (c->kernel == TWMC_KERNEL_BAND) {
     c->block = 64;
     c->lds = 0;
-    HIPCHK(c, band_occupancy(c->n, c->band_w, &per_cu));
+    HIPCHK(c, band_occupancy(c->n, c->band_w, c->band_lieb, &per_cu));
   } else if (c->kernel == TWMC_KERNEL_MFMA) {
     c->block = 256;
     c->lds = mfma_lds_bytes(c->np);

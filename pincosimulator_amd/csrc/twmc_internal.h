@@ -218,7 +218,7 @@ void band_coefficients(const double* coupling, int n, int w, double dt, float* o
 // bonds (A: +1; B: -2, -1, +1; C: -1, +2, by site index mod 3) and S = 6, W = 2
 bool band_lieb_pattern(const float* coef, int n, int w);
 hipError_t launch_band(const StepArgs& a, int grid, hipStream_t s);
-hipError_t band_occupancy(int n, int w, int* blocks_per_cu);
+hipError_t band_occupancy(int n, int w, int lieb, int* blocks_per_cu);
 hipError_t mfma_occupancy(int np, int* blocks_per_cu);
 // splits A (row-major np x np, FP64) into BF16 hi and lo parts (raw 16-bit)
 void mfma_split(const double* a, size_t count, uint16_t* hi, uint16_t* lo);
