@@ -40,9 +40,14 @@ __device__ __forceinline__ float rot_down(float v) {
 }
 
 // UI0: Im U = 0 (a real interaction), so the real part of the drift factor is
-// the constant A0 and its per-site FMA drops out
-template <bool UI0>
-__global__ void __launch_bounds__(64, 2) stencil64_step_kernel(StepArgs g) {
+// the constant A0 and its per-site FMA drops out.  R: sites y < R of the row
+// live in registers, the other kL - R in LDS ([y - R][lane], 512 bytes per
+// site and wave, conflict-free); with R = 46 the kernel fits four waves per
+// SIMD, so a batch of 16 trajectories per SIMD runs in four full rounds
+template <bool UI0, int R>
+__global__ void __launch_bounds__(64, R < kL ? 4 : 2) stencil64_step_kernel(StepArgs g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<float>* const row_lds = reinterpret_cast<cpx<float>*>(smem);  // [kL - R][64]
   const int lane = threadIdx.x;  // row x
 
   // uniform site parameters (the kernel is chosen only when they are uniform)
@@ -62,13 +67,23 @@ __global__ void __launch_bounds__(64, 2) stencil64_step_kernel(StepArgs g) {
   for (int b = blockIdx.x; b < g.n_traj; b += gridDim.x) {
     const uint64_t traj = g.traj_ids[b];
     cpx<float>* gb = reinterpret_cast<cpx<float>*>(g.beta) + (size_t)b * 4096 + (size_t)lane * kL;
-    cpx<float> beta[kL];
+    cpx<float> beta[R];
+    // site y of this lane's row: a register below R, else its LDS slot
+    auto at = [&](int y) __attribute__((always_inline)) {
+      return y < R ? beta[y < R ? y : 0] : row_lds[(y - R) * 64 + lane];
+    };
+    auto put = [&](int y, cpx<float> v) __attribute__((always_inline)) {
+      if (y < R)
+        beta[y < R ? y : 0] = v;
+      else
+        row_lds[(y - R) * 64 + lane] = v;
+    };
     // lane x reads its own row: 512 contiguous bytes, 16-byte loads
 #pragma unroll
     for (int y = 0; y < kL; y += 2) {
       const float4 v = *reinterpret_cast<const float4*>(gb + y);
-      beta[y] = mk(v.x, v.y);
-      beta[y + 1] = mk(v.z, v.w);
+      put(y, mk(v.x, v.y));
+      put(y + 1, mk(v.z, v.w));
     }
 
     for (int s = 0; s < g.n_steps; ++s) {
@@ -76,11 +91,14 @@ __global__ void __launch_bounds__(64, 2) stencil64_step_kernel(StepArgs g) {
       uint32_t pbase = (uint32_t)lane * 32u;
       asm volatile("" : "+v"(pbase));
       const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
-      const cpx<float> first = beta[0];  // old value of the y + 1 wrap of y = 63
+      const cpx<float> first = at(0);  // old value of the y + 1 wrap of y = 63
       // y-neighbour sum of the next site, formed from old values before the
       // site between them is overwritten (so each new value can take its old
       // value's register)
-      float syr = beta[kL - 1].re + beta[1].re, syi = beta[kL - 1].im + beta[1].im;
+      const cpx<float> last = at(kL - 1), second = at(1);
+      float syr = last.re + second.re, syi = last.im + second.im;
+      // LDS sites read for a y-sum two sites ahead are kept for their own turn
+      cpx<float> ahead[2] = {second, second};
 #pragma unroll
       for (int p = 0; p < kL / 2; ++p) {
         const u32x4 w = philox4x32_10(make_counter(traj, TAG_WIENER, step, pbase + (uint32_t)p), g.key0, g.key1);
@@ -90,12 +108,13 @@ __global__ void __launch_bounds__(64, 2) stencil64_step_kernel(StepArgs g) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int y = 2 * p + h;
-          const cpx<float> v = beta[y];
+          const cpx<float> v = (y >= R && y >= 2) ? ahead[y & 1] : at(y);
           // every lane reads its neighbours' old beta[y] before any lane writes it
           const float sr = rot_down(v.re) + (rot_up(v.re) + syr);
           const float si = rot_down(v.im) + (rot_up(v.im) + syi);
           if (y < kL - 1) {
-            const cpx<float> y2 = (y + 2 == kL) ? first : beta[y + 2];
+            const cpx<float> y2 = (y + 2 == kL) ? first : (y + 2 == kL - 1) ? last : at(y + 2 < kL ? y + 2 : 0);
+            if (y + 2 >= R && y + 2 < kL) ahead[y & 1] = y2;
             syr = v.re + y2.re;
             syi = v.im + y2.im;
           }
@@ -110,14 +129,16 @@ __global__ void __launch_bounds__(64, 2) stencil64_step_kernel(StepArgs g) {
           di = fmaf(cr, v.im, di);
           di = fmaf(ci, v.re, di);
           di = fmaf(-jdt, sr, di);
-          beta[y] = mk(v.re + dr, v.im + di);
+          put(y, mk(v.re + dr, v.im + di));
         }
       }
     }
 
 #pragma unroll
-    for (int y = 0; y < kL; y += 2)
-      *reinterpret_cast<float4*>(gb + y) = make_float4(beta[y].re, beta[y].im, beta[y + 1].re, beta[y + 1].im);
+    for (int y = 0; y < kL; y += 2) {
+      const cpx<float> v0 = at(y), v1 = at(y + 1);
+      *reinterpret_cast<float4*>(gb + y) = make_float4(v0.re, v0.im, v1.re, v1.im);
+    }
   }
 }
 
@@ -127,16 +148,21 @@ bool stencil64_supported(int nx, int ny, int hopping, int precision, int param_s
   return nx == 64 && ny == 64 && hopping == 0 && precision == 32 && param_stride == 0;
 }
 
+// sites per row kept in registers (the rest in LDS)
+constexpr int kRegSites = 46;
+constexpr size_t kRowLds = (size_t)(kL - kRegSites) * 64 * sizeof(cpx<float>);
+
 hipError_t launch_stencil64(const StepArgs& a, int grid, hipStream_t s) {
   if (a.u_real)
-    hipLaunchKernelGGL(stencil64_step_kernel<true>, dim3(grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((stencil64_step_kernel<true, kRegSites>), dim3(grid), dim3(64), kRowLds, s, a);
   else
-    hipLaunchKernelGGL(stencil64_step_kernel<false>, dim3(grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((stencil64_step_kernel<false, kRegSites>), dim3(grid), dim3(64), kRowLds, s, a);
   return hipGetLastError();
 }
 
 hipError_t stencil64_occupancy(int* blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)stencil64_step_kernel<false>, 64, 0);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)stencil64_step_kernel<false, kRegSites>,
+                                                      64, kRowLds);
 }
 
 }  // namespace twmc
