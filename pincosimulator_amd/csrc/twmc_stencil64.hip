@@ -11,13 +11,14 @@
 // with periodic wrap -- the same linear map, without the FFT round trip (the
 // two agree to rounding, ~1e-16 relative in FP64).  This kernel evaluates that
 // form: one wavefront owns one trajectory, lane x holds row x (64 complex
-// values in registers); y-neighbours are neighbouring registers, x-neighbours
-// come from lanes x-1 and x+1 (whole-wave DPP rotations).  Per step and site:
-// the 4-term sum, the local drift, one Philox4x32-10 call per site pair with
+// values, most in registers, the last few in the wave's LDS slice);
+// y-neighbours are neighbouring sites of the row, x-neighbours come from
+// lanes x-1 and x+1 (whole-wave DPP rotations).  Per step and site: the
+// 4-term sum, the local drift, one Philox4x32-10 call per site pair with
 // Box-Muller, the update.  Frames are captured by separate launches (the
 // register-FFT capture kernel for the per-trajectory means and the k-space
 // sums, the site-major kernel for the per-site sums), so this loop carries no
-// capture code and no LDS at all.
+// capture code.
 #include "twmc_cplx.h"
 #include "twmc_internal.h"
 #include "twmc_rng.h"
