@@ -86,16 +86,28 @@ def cpu_baseline(threads, spec=None, target_s=12.0):
     kind = ("sparse coupling product" if spec.get("lattice") == "matrix"
             else "FFT hopping with the oracle's own FFT, FFTW not being installed")
     cpu = "unknown CPU"
+    topo = ""
     try:
+        # model and topology as lscpu reports them: sockets (physical ids),
+        # cores per socket ("cpu cores"), threads per core (siblings / cores)
         with open("/proc/cpuinfo") as f:
-            cpu = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), cpu)
-    except OSError:
+            lines = f.read().splitlines()
+        field = lambda name: [l.split(":", 1)[1].strip() for l in lines if l.split(":", 1)[0].strip() == name]
+        cpu = next(iter(field("model name")), cpu)
+        sockets = len(set(field("physical id"))) or 1
+        cores = int(next(iter(field("cpu cores")), "0") or 0)
+        siblings = int(next(iter(field("siblings")), "0") or 0)
+        usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        if cores:
+            topo = (f"{sockets} socket(s) x {cores} cores x {max(1, siblings // cores)} threads per core; "
+                    f"{usable} CPUs usable by this process; ")
+    except (OSError, ValueError):
         pass
     return {
         "value": value, "unit": "trajectory·site-steps/s", "cores": threads, "kind": "port",
         "sample": f"{n_traj} trajectories x {steps} steps of the same {n_sites}-site system "
                   f"({t:.1f} s wall), FP64, reference per-step structure ({kind}) with std::mt19937 "
-                  f"noise, one trajectory per task on {threads} std::threads ({cpu}; {os.cpu_count()} logical CPUs "
+                  f"noise, one trajectory per task on {threads} std::threads ({cpu}; {topo}{os.cpu_count()} logical CPUs "
                   f"visible, threads = min(16, logical CPUs))",
     }
 
