@@ -112,6 +112,21 @@ def cpu_baseline(threads, spec=None, target_s=12.0):
     }
 
 
+def physical_cores(threads):
+    """Physical cores among `threads` logical CPUs, from the threads per core
+    that /proc/cpuinfo reports (siblings / cpu cores)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            lines = f.read().splitlines()
+        field = lambda name: [l.split(":", 1)[1].strip() for l in lines if l.split(":", 1)[0].strip() == name]
+        cores = int(next(iter(field("cpu cores")), "0") or 0)
+        siblings = int(next(iter(field("siblings")), "0") or 0)
+        per_core = max(1, siblings // cores) if cores else 1
+    except (OSError, ValueError):
+        per_core = 1
+    return max(1, threads // per_core)
+
+
 SWEEP_SHAPES = [(1, 32), (1, 64), (1, 128), (1, 256), (1, 512), (16, 16), (32, 32), (48, 48), (64, 64)]
 
 
@@ -401,13 +416,18 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         log(f"bench: CPU baseline on {threads} threads")
         out["cpu_baseline"] = cpu_baseline(threads, base)
-        if threads > 1:
-            # the same sample on one thread per physical core (2-way SMT)
-            half = threads // 2
-            log(f"bench: CPU baseline on {half} threads")
-            b = cpu_baseline(half, base, target_s=8.0)
-            out["cpu_baseline"]["per_physical_core_run"] = {"value": b["value"], "cores": half,
+        # the same sample on one thread per physical core, when SMT leaves
+        # fewer physical cores than logical CPUs among those used
+        phys = physical_cores(threads)
+        if phys < threads:
+            log(f"bench: CPU baseline on {phys} threads (one per physical core)")
+            b = cpu_baseline(phys, base, target_s=8.0)
+            out["cpu_baseline"]["per_physical_core_run"] = {"value": b["value"], "cores": phys,
                                                             "sample": b["sample"]}
+        else:
+            out["cpu_baseline"]["per_physical_core_run"] = {
+                "value": out["cpu_baseline"]["value"], "cores": threads,
+                "sample": "the run above: /proc/cpuinfo shows one thread per core, so its threads are physical cores"}
     print(json.dumps(out), flush=True)
 
 
