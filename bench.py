@@ -127,21 +127,9 @@ def physical_cores(threads):
     return max(1, threads // per_core)
 
 
-SWEEP_SHAPES = [(1, 32), (1, 64), (1, 128), (1, 256), (1, 512), (16, 16), (32, 32), (48, 48), (64, 64)]
-
-
 def sweep_points():
-    """512 points on a 32 x 16 grid of F in [0.5, 4] and detuning in [-2, 3],
-    shapes assigned round-robin, 128 trajectories each, U=0.05, gamma=1, J=1."""
-    import numpy as np
-    Fs = np.linspace(0.5, 4.0, 32)
-    Ds = np.linspace(-2.0, 3.0, 16)
-    pts = []
-    for k in range(512):
-        nx, ny = SWEEP_SHAPES[k % len(SWEEP_SHAPES)]
-        pts.append(dict(nx=nx, ny=ny, n_traj=128, traj_id0=128 * k, J=1.0, gamma=1.0,
-                        omega=float(Ds[k // 32]), U=0.05, F=float(Fs[k % 32]), beta_init=0.0, beta_sigma=0.5))
-    return pts
+    from pincosimulator_amd.workloads import sweep512_points
+    return sweep512_points()
 
 
 def sweep_traffic():
@@ -276,6 +264,77 @@ def time_frames(eng, n_traj, W, K):
     return eng, info, st, t1 - t0
 
 
+PEAK_FP64 = 78.6e12            # MI355X FP64 vector peak (FLOP/s)
+
+
+def measure_engine(base, n_traj, fsteps, kernel, precision, W, K):
+    """W untimed and K timed frames of one Engine workload; returns
+    (site-steps/s over the wall clock, average step-kernel launch in ms, info)."""
+    import numpy as np
+    from pincosimulator_amd import Engine, System
+    spec = dict(base, t_end=(W + K) * fsteps * base["dt"])
+    eng, info, st, wall = time_frames(Engine(System(**spec), precision=precision, kernel=kernel), n_traj, W, K)
+    res = eng.read()
+    eng.close()
+    if not np.all(np.isfinite(res.sum_n)):
+        sys.exit("non-finite ensemble sums")
+    site_steps = float(n_traj) * base["nx"] * base["ny"] * fsteps * K
+    return site_steps / wall, st["step_kernel_ms"] / max(st["step_launches"], 1), info
+
+
+def secondary_lines():
+    """The other measured rows, in the default line: the 64x64 workload in FP64
+    (the reference's precision), the Lieb chain on MFMA and on the banded
+    kernel, and the 512-point sweep."""
+    out = {}
+    v, ms, info = measure_engine(SPEC, N_TRAJ, FRAME_STEPS, "generic", 64, 1, 2)
+    a = float(N_TRAJ) * 4096 * FRAME_STEPS * FLOPS_PER_SITE_STEP / (ms * 1e-3)
+    log(f"bench: FP64 64x64: {v:.4e} site-steps/s, step kernel avg {ms:.2f} ms")
+    out["fp64"] = {
+        "value": v, "unit": "trajectory·site-steps/s", "dtype": "f64", "kernel": info["kernel"] + "_step_kernel",
+        "workload": "the 64x64 workload above (16384 trajectories, 500-step frames, 2 timed) with an FP64 state, "
+                    "the reference's FFT form every step",
+        "roofline": {"bound": "valu", "achieved": a / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": a / PEAK_FP64, "avg_launch_ms": ms,
+                     "flops_basis": "algorithmic: 150 flops per site-step, as the line's roofline"}}
+    lieb = lieb_spec()
+    n = lieb["nx"] * lieb["ny"]
+    v, ms, _ = measure_engine(lieb, LIEB_TRAJ, LIEB_FRAME, "mfma", 32, 1, 10)
+    a = float(LIEB_TRAJ) * n * LIEB_FRAME * 3 * 4.0 * n / (ms * 1e-3)
+    vb, msb, _ = measure_engine(lieb, LIEB_TRAJ, LIEB_FRAME, "band", 32, 1, 10)
+    log(f"bench: lieb384: MFMA {v:.4e} site-steps/s ({ms:.3f} ms per launch), band {vb:.4e} ({msb:.3f} ms)")
+    out["lieb384"] = {
+        "workload": WORKLOAD_TEXT["lieb384"] + "; 10 timed frames",
+        "mfma": {"value": v, "unit": "trajectory·site-steps/s", "kernel": "mfma_step_kernel", "avg_launch_ms": ms,
+                 "roofline": {"bound": "mfma", "achieved": a / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                              "frac": a / PEAK_BF16,
+                              "flops_basis": "executed BF16 MFMA flops: 3 products x 4 N per site-step"}},
+        "band": {"value": vb, "unit": "trajectory·site-steps/s", "kernel": "band_step_kernel", "avg_launch_ms": msb}}
+    import math
+    from pincosimulator_amd import Sweep
+    pts = sweep_points()
+    sw = Sweep(pts, 0.002, 500, 20260103)
+    sw.autotune(steps=50)
+    sw.run(500 * 0.002, graph=True)
+    K = 10
+    t0 = time.perf_counter()
+    r = sw.run(K * 500 * 0.002, graph=True)
+    wall = time.perf_counter() - t0
+    sw.close()
+    ss = sum(p["n_traj"] * p["nx"] * p["ny"] for p in pts) * 500.0 * K
+    fl = sum(p["n_traj"] * p["nx"] * p["ny"] * (10.0 * math.log2(p["nx"] * p["ny"]) + 30.0) for p in pts) * 500.0 * K
+    a = fl / (r["kernel_ms"] * 1e-3)
+    log(f"bench: sweep512: {ss / wall:.4e} site-steps/s over {K} frames")
+    out["sweep512"] = {
+        "value": ss / wall, "unit": "trajectory·site-steps/s", "kernel": "sweep_step_kernel",
+        "workload": "512-point sweep (shapes round-robin, 128 trajectories each), stacking autotuned first, "
+                    f"graph-captured frame loop, {K} timed 500-step frames",
+        "roofline": {"bound": "valu", "achieved": a / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": a / PEAK_FP32, "traffic": sweep_traffic(),
+                     "flops_basis": "algorithmic: 10 log2 N + 30 flops per site-step of each point's lattice"}}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,6 +346,8 @@ def main():
                     help="held-out points of the warm-start leg (square64) or evaluation (warmstart64)")
     ap.add_argument("--ws-traj", type=int, default=16384)
     ap.add_argument("--no-warmstart", action="store_true", help="square64: skip the warm-start leg")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="square64: skip the FP64, Lieb-chain and sweep rows of the line")
     ap.add_argument("--kernel", default="auto", choices=["auto", "stencil64", "fused64", "generic", "mfma", "band"],
                     help="step kernel (auto: the fastest one that supports the workload)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -409,6 +470,8 @@ def main():
             "value": site_steps / w2, "ms_per_step": w2 / K * 1e3, "avg_launch_ms": b_ms,
             "note": "the same workload with the coupling applied as its (2W+1)-point band stencil (W = 2), "
                     "the library's AUTO choice for banded couplings"}
+    if args.workload == "square64" and not args.no_secondary:
+        out.update(secondary_lines())
     if args.workload == "square64" and not args.no_warmstart and args.ws_points > 0:
         log(f"bench: warm-start leg, {args.ws_points} held-out points")
         out["time_to_steady_state"] = warmstart_summary(args.ws_points, args.ws_traj)

@@ -47,6 +47,12 @@ extern "C" {
 
 #define TWMC_ABI_VERSION 1
 
+/* Opaque handles, one type per kind of context, so that passing a sweep or
+ * network context where an engine context is expected does not compile. */
+typedef struct twmc_ctx twmc_ctx;             /* one parameter set on one GPU */
+typedef struct twmc_sweep_ctx twmc_sweep_ctx; /* a batched parameter sweep */
+typedef struct twmc_mlp_ctx twmc_mlp_ctx;     /* the warm-start network */
+
 enum {
   TWMC_OK = 0,
   TWMC_ERR_INVALID = -1,     /* bad argument or unsupported combination */
@@ -166,22 +172,25 @@ typedef struct twmc_stats {
 } twmc_stats;
 
 int twmc_abi_version(void);
-const char* twmc_last_error(const void* ctx);  /* ctx may be NULL: last global error */
+const char* twmc_last_error(const twmc_ctx* ctx);  /* ctx may be NULL: last global error */
 
-int twmc_create(const twmc_params* p, int device, void** ctx_out);
-void twmc_destroy(void* ctx);
+int twmc_create(const twmc_params* p, int device, twmc_ctx** ctx_out);
+void twmc_destroy(twmc_ctx* ctx);
 
-int twmc_batch_begin(void* ctx, const twmc_batch_desc* d);
-int twmc_batch_info_get(void* ctx, twmc_batch_info* info);
-int twmc_batch_advance(void* ctx, uint64_t n_segments);   /* asynchronous */
-int twmc_batch_sync(void* ctx);
-int twmc_batch_read(void* ctx, twmc_output* out);         /* synchronises */
-int twmc_run(void* ctx, const twmc_batch_desc* d, twmc_output* out);
+int twmc_batch_begin(twmc_ctx* ctx, const twmc_batch_desc* d);
+int twmc_batch_info_get(twmc_ctx* ctx, twmc_batch_info* info);
+int twmc_batch_advance(twmc_ctx* ctx, uint64_t n_segments);   /* asynchronous */
+int twmc_batch_sync(twmc_ctx* ctx);
+int twmc_batch_read(twmc_ctx* ctx, twmc_output* out);         /* synchronises */
+int twmc_run(twmc_ctx* ctx, const twmc_batch_desc* d, twmc_output* out);
 
-int twmc_set_timing(void* ctx, int enable);   /* HIP events around step launches */
-int twmc_query(void* ctx, twmc_stats* st);
-int twmc_reset_stats(void* ctx);
-float twmc_progress(const void* ctx);         /* fraction of the batch's steps enqueued; any thread may call it */
+int twmc_set_timing(twmc_ctx* ctx, int enable);   /* HIP events around step launches */
+int twmc_query(twmc_ctx* ctx, twmc_stats* st);
+int twmc_reset_stats(twmc_ctx* ctx);
+/* fraction of the current batch's steps the device has completed (an event
+ * after each segment, polled without blocking); any thread may call it while
+ * the owning thread advances or reads the batch */
+float twmc_progress(const twmc_ctx* ctx);
 
 /* Host-side NoisyMatrix realisation with the kernels' Philox layout (tag as
  * in twmc_rng.h).  Used for one shared disorder realisation. */
@@ -203,15 +212,15 @@ typedef struct twmc_sweep_point {
 } twmc_sweep_point;
 
 int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt, uint64_t frame_steps,
-                      uint64_t seed, int device, void** ctx_out);
-void twmc_sweep_destroy(void* ctx);
-const char* twmc_sweep_last_error(const void* ctx);
+                      uint64_t seed, int device, twmc_sweep_ctx** ctx_out);
+void twmc_sweep_destroy(twmc_sweep_ctx* ctx);
+const char* twmc_sweep_last_error(const twmc_sweep_ctx* ctx);
 /* trajectories stacked per step work item, per point (tb x sites <= 4096) */
-int twmc_sweep_set_batching(void* ctx, const int32_t* tb);
-int twmc_sweep_get_batching(void* ctx, int32_t* tb);
+int twmc_sweep_set_batching(twmc_sweep_ctx* ctx, const int32_t* tb);
+int twmc_sweep_get_batching(twmc_sweep_ctx* ctx, int32_t* tb);
 /* times the stacking candidates of every lattice shape for `steps` steps and
  * keeps the fastest; tb_out (n_points, may be NULL) receives the choice */
-int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out);
+int twmc_sweep_autotune(twmc_sweep_ctx* ctx, int32_t steps, int32_t* tb_out);
 /* vacuum-start run from t_start to t_end with the stored initial frame;
  * traj_m2 / traj_m4 (total trajectories x n_frames, points in order) receive
  * the per-trajectory site means of |beta|^2 and |beta|^4; ksum (n_points x
@@ -221,12 +230,12 @@ int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out);
  * HIP-event time.  Any output pointer may be NULL.  The run stores at most
  * nearbyint((t_end - t_start) / (dt frame_steps)) + 1 frames (n_frames_out
  * the actual count): size the outputs for that many. */
-int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, double* traj_m2, double* traj_m4,
+int twmc_sweep_run(twmc_sweep_ctx* ctx, double t_start, double t_end, int32_t use_graph, double* traj_m2, double* traj_m4,
                    double* ksum, uint64_t* n_frames_out, double* kernel_ms);
 /* radial bin (0..31) of every mode of point `point` (nx * ny entries, row-major):
  * bin 0 is k = 0, others 1 + floor(|k| / (pi sqrt 2 / 31)) capped at 31 */
-int twmc_sweep_mode_bins(void* ctx, uint64_t point, uint8_t* bins);
-int twmc_sweep_query(void* ctx, twmc_stats* st);
+int twmc_sweep_mode_bins(twmc_sweep_ctx* ctx, uint64_t point, uint8_t* bins);
+int twmc_sweep_query(twmc_sweep_ctx* ctx, twmc_stats* st);
 
 /* ---- learned warm start: the network --------------------------------------
  * A fully connected MLP (n_in -> n_hidden_layers x n_hidden, exact-erf GELU ->
@@ -247,17 +256,17 @@ typedef struct twmc_mlp_train_opts {
   uint64_t shuffle_seed;
 } twmc_mlp_train_opts;
 
-int twmc_mlp_create(const twmc_mlp_config* cfg, int device, void** ctx_out);
-void twmc_mlp_destroy(void* ctx);
-const char* twmc_mlp_last_error(const void* ctx);
-uint64_t twmc_mlp_n_params(const void* ctx);
-int twmc_mlp_get_params(void* ctx, float* out);
-int twmc_mlp_set_params(void* ctx, const float* in);   /* also clears the Adam moments */
-int twmc_mlp_forward(void* ctx, const float* x, uint64_t rows, float* y);
-int twmc_mlp_loss_grad(void* ctx, const float* x, const float* t, const float* w, uint64_t rows, double* loss,
+int twmc_mlp_create(const twmc_mlp_config* cfg, int device, twmc_mlp_ctx** ctx_out);
+void twmc_mlp_destroy(twmc_mlp_ctx* ctx);
+const char* twmc_mlp_last_error(const twmc_mlp_ctx* ctx);
+uint64_t twmc_mlp_n_params(const twmc_mlp_ctx* ctx);
+int twmc_mlp_get_params(twmc_mlp_ctx* ctx, float* out);
+int twmc_mlp_set_params(twmc_mlp_ctx* ctx, const float* in);   /* also clears the Adam moments */
+int twmc_mlp_forward(twmc_mlp_ctx* ctx, const float* x, uint64_t rows, float* y);
+int twmc_mlp_loss_grad(twmc_mlp_ctx* ctx, const float* x, const float* t, const float* w, uint64_t rows, double* loss,
                        float* grad);
 /* epoch_loss (epochs entries, may be NULL): mean minibatch loss per epoch */
-int twmc_mlp_train(void* ctx, const float* x, const float* t, const float* w, uint64_t rows,
+int twmc_mlp_train(twmc_mlp_ctx* ctx, const float* x, const float* t, const float* w, uint64_t rows,
                    const twmc_mlp_train_opts* opts, double* epoch_loss);
 
 #ifdef __cplusplus

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -88,8 +89,14 @@ struct Ctx {
   std::vector<Segment> segs;
   size_t next_seg = 0;
   uint64_t steps_done = 0;
-  // fraction of the batch's steps enqueued, read by other threads (twmc_progress)
-  std::atomic<float> progress{0.f};
+  // completion of the batch on the device, read by other threads
+  // (twmc_progress): one event recorded after each segment's launches, the
+  // steps done at that point, and how many of them have fired
+  mutable std::mutex prog_mu;
+  std::vector<hipEvent_t> seg_ev;
+  std::vector<uint64_t> seg_steps;
+  size_t seg_recorded = 0;
+  mutable size_t seg_completed = 0;
   int grid = 0, block = 256;
   int cap_grid = 0;  // stencil64: grid of the capture launch (the 64x64 FFT kernel's)
   size_t lds = 0;
@@ -411,12 +418,12 @@ extern "C" {
 
 int twmc_abi_version(void) { return TWMC_ABI_VERSION; }
 
-const char* twmc_last_error(const void* ctx) {
-  if (ctx) return static_cast<const Ctx*>(ctx)->err.c_str();
+const char* twmc_last_error(const twmc_ctx* ctx) {
+  if (ctx) return reinterpret_cast<const Ctx*>(ctx)->err.c_str();
   return g_last_error.c_str();
 }
 
-int twmc_create(const twmc_params* p, int device, void** ctx_out) {
+int twmc_create(const twmc_params* p, int device, twmc_ctx** ctx_out) {
   if (!p || !ctx_out) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
   *ctx_out = nullptr;
   if (p->nx < 1 || p->ny < 1) return fail(nullptr, TWMC_ERR_INVALID, "nx and ny must be >= 1");
@@ -612,16 +619,16 @@ int twmc_create(const twmc_params* p, int device, void** ctx_out) {
   }
   if (fail_rc != TWMC_OK) {
     std::string m = c->err;
-    twmc_destroy(c);
+    twmc_destroy(reinterpret_cast<twmc_ctx*>(c));
     return fail(nullptr, fail_rc, m);
   }
-  *ctx_out = c;
+  *ctx_out = reinterpret_cast<twmc_ctx*>(c);
   return TWMC_OK;
 }
 
-void twmc_destroy(void* ctx) {
+void twmc_destroy(twmc_ctx* ctx) {
   if (!ctx) return;
-  Ctx* c = static_cast<Ctx*>(ctx);
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   stack_release(c->stack);
@@ -633,6 +640,7 @@ void twmc_destroy(void* ctx) {
     (void)hipEventDestroy(pr.first);
     (void)hipEventDestroy(pr.second);
   }
+  for (hipEvent_t e : c->seg_ev) (void)hipEventDestroy(e);
   DevBuf* bufs[] = {&c->d_twx,       &c->d_twy,      &c->d_kcx,       &c->d_kcy,  &c->d_cscp, &c->d_cscr,
                     &c->d_cscv,      &c->d_perm_site, &c->d_perm_k,   &c->d_ahi,  &c->d_alo,  &c->d_kamp,
                     &c->d_ft_times,  &c->d_ft_vals,  &c->d_ft_slopes, &c->d_band, &c->d_ids,  &c->d_beta,
@@ -647,9 +655,9 @@ void twmc_destroy(void* ctx) {
   delete c;
 }
 
-int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
+int twmc_batch_begin(twmc_ctx* ctx, const twmc_batch_desc* d) {
   if (!ctx || !d) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  Ctx* c = static_cast<Ctx*>(ctx);
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (d->n_traj == 0 || !d->traj_ids) return fail(c, TWMC_ERR_INVALID, "empty trajectory batch");
   if (d->n_traj > (uint64_t)INT32_MAX) return fail(c, TWMC_ERR_INVALID, "batch too large");
   if (d->init_mode < 0 || d->init_mode > 3) return fail(c, TWMC_ERR_INVALID, "unknown init mode");
@@ -679,7 +687,17 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
   plan_segments(c);
   c->next_seg = 0;
   c->steps_done = 0;
-  c->progress.store(0.f, std::memory_order_relaxed);
+  {
+    std::lock_guard<std::mutex> lk(c->prog_mu);
+    while (c->seg_ev.size() < c->segs.size()) {
+      hipEvent_t e = nullptr;
+      HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      c->seg_ev.push_back(e);
+    }
+    c->seg_steps.assign(c->segs.size(), 0);
+    c->seg_recorded = 0;
+    c->seg_completed = 0;
+  }
 
   const int n = c->n;
   const size_t B = d->n_traj;
@@ -743,7 +761,11 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
   HIPCHK(c, c->d_rsl.ensure(sets * n * c->es));
   HIPCHK(c, c->d_U.ensure(sets * n * c->es));
   HIPCHK(c, c->d_F.ensure(sets * n * c->es));
-  size_t part_bytes = (size_t)std::max(c->grid, c->cap_grid) * kQ * n * sizeof(double);
+  // slot rows: one per step-kernel workgroup, per capture workgroup, and per
+  // trajectory chunk of site_sums_kernel (stencil and banded paths)
+  int ss_chunk = 0;
+  const int slots = std::max({c->grid, c->cap_grid, site_sum_chunks(B, kSiteSumChunks, &ss_chunk)});
+  size_t part_bytes = (size_t)slots * kQ * n * sizeof(double);
   HIPCHK(c, c->d_partials.ensure(part_bytes));
   HIPCHK(c, hipMemsetAsync(c->d_partials.p, 0, part_bytes, s));
   size_t nf = std::max<uint64_t>(c->n_frames, 1);
@@ -803,9 +825,9 @@ int twmc_batch_begin(void* ctx, const twmc_batch_desc* d) {
   return TWMC_OK;
 }
 
-int twmc_batch_info_get(void* ctx, twmc_batch_info* info) {
+int twmc_batch_info_get(twmc_ctx* ctx, twmc_batch_info* info) {
   if (!ctx || !info) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  Ctx* c = static_cast<Ctx*>(ctx);
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c->batch_active) return fail(c, TWMC_ERR_STATE, "no active batch");
   info->n_frames = c->n_frames;
   info->n_steps = c->n_steps;
@@ -817,9 +839,9 @@ int twmc_batch_info_get(void* ctx, twmc_batch_info* info) {
   return TWMC_OK;
 }
 
-int twmc_batch_advance(void* ctx, uint64_t n_segments) {
+int twmc_batch_advance(twmc_ctx* ctx, uint64_t n_segments) {
   if (!ctx) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  Ctx* c = static_cast<Ctx*>(ctx);
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c->batch_active) return fail(c, TWMC_ERR_STATE, "no active batch");
   HIPCHK(c, hipSetDevice(c->device));
   hipStream_t s = c->stream;
@@ -933,27 +955,31 @@ int twmc_batch_advance(void* ctx, uint64_t n_segments) {
         }
       }
       c->steps_done += (uint64_t)steps;
-      if (c->n_steps > 0)
-        c->progress.store(float(double(c->steps_done) / double(c->n_steps)), std::memory_order_relaxed);
       left -= (uint64_t)steps;
       first = false;
     } while (left > 0);
+    {
+      std::lock_guard<std::mutex> lk(c->prog_mu);
+      HIPCHK(c, hipEventRecord(c->seg_ev[c->next_seg], s));
+      c->seg_steps[c->next_seg] = c->steps_done;
+      c->seg_recorded = c->next_seg + 1;
+    }
     c->next_seg++;
   }
   return TWMC_OK;
 }
 
-int twmc_batch_sync(void* ctx) {
+int twmc_batch_sync(twmc_ctx* ctx) {
   if (!ctx) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  Ctx* c = static_cast<Ctx*>(ctx);
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TWMC_OK;
 }
 
-int twmc_batch_read(void* ctx, twmc_output* out) {
+int twmc_batch_read(twmc_ctx* ctx, twmc_output* out) {
   if (!ctx || !out) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  Ctx* c = static_cast<Ctx*>(ctx);
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c->batch_active) return fail(c, TWMC_ERR_STATE, "no active batch");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1000,10 +1026,10 @@ int twmc_batch_read(void* ctx, twmc_output* out) {
   return TWMC_OK;
 }
 
-int twmc_run(void* ctx, const twmc_batch_desc* d, twmc_output* out) {
+int twmc_run(twmc_ctx* ctx, const twmc_batch_desc* d, twmc_output* out) {
   int rc = twmc_batch_begin(ctx, d);
   if (rc) return rc;
-  Ctx* c = static_cast<Ctx*>(ctx);
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
   rc = twmc_batch_advance(ctx, c->segs.size());
   if (rc) return rc;
   if (out) rc = twmc_batch_read(ctx, out);
@@ -1011,15 +1037,15 @@ int twmc_run(void* ctx, const twmc_batch_desc* d, twmc_output* out) {
   return rc;
 }
 
-int twmc_set_timing(void* ctx, int enable) {
+int twmc_set_timing(twmc_ctx* ctx, int enable) {
   if (!ctx) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  static_cast<Ctx*>(ctx)->timing = enable != 0;
+  reinterpret_cast<Ctx*>(ctx)->timing = enable != 0;
   return TWMC_OK;
 }
 
-int twmc_query(void* ctx, twmc_stats* st) {
+int twmc_query(twmc_ctx* ctx, twmc_stats* st) {
   if (!ctx || !st) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  Ctx* c = static_cast<Ctx*>(ctx);
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (auto& pr : c->ev_step) {
@@ -1048,18 +1074,23 @@ int twmc_query(void* ctx, twmc_stats* st) {
   return TWMC_OK;
 }
 
-int twmc_reset_stats(void* ctx) {
+int twmc_reset_stats(twmc_ctx* ctx) {
   if (!ctx) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  Ctx* c = static_cast<Ctx*>(ctx);
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
   twmc_stats z{};
   twmc_query(ctx, &z);
   c->stats = twmc_stats{};
   return TWMC_OK;
 }
 
-float twmc_progress(const void* ctx) {
+float twmc_progress(const twmc_ctx* ctx) {
   if (!ctx) return 0.f;
-  return static_cast<const Ctx*>(ctx)->progress.load(std::memory_order_relaxed);
+  const Ctx* c = reinterpret_cast<const Ctx*>(ctx);
+  std::lock_guard<std::mutex> lk(c->prog_mu);
+  while (c->seg_completed < c->seg_recorded && hipEventQuery(c->seg_ev[c->seg_completed]) == hipSuccess)
+    c->seg_completed++;
+  if (c->seg_completed == 0 || c->n_steps == 0) return 0.f;
+  return float(double(c->seg_steps[c->seg_completed - 1]) / double(c->n_steps));
 }
 
 int twmc_noisy_matrix(const twmc_matrix* m, uint64_t n, uint64_t seed, uint64_t traj, uint32_t tag,

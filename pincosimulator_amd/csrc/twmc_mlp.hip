@@ -347,12 +347,12 @@ using namespace twmc;
 
 extern "C" {
 
-const char* twmc_mlp_last_error(const void* ctx) {
-  if (ctx) return static_cast<const MlpCtx*>(ctx)->err.c_str();
+const char* twmc_mlp_last_error(const twmc_mlp_ctx* ctx) {
+  if (ctx) return reinterpret_cast<const MlpCtx*>(ctx)->err.c_str();
   return g_mlp_err.c_str();
 }
 
-int twmc_mlp_create(const twmc_mlp_config* cfg, int device, void** ctx_out) {
+int twmc_mlp_create(const twmc_mlp_config* cfg, int device, twmc_mlp_ctx** ctx_out) {
   if (!cfg || !ctx_out) return mlp_fail(nullptr, TWMC_ERR_INVALID, "null argument");
   *ctx_out = nullptr;
   if (cfg->n_in < 1 || cfg->n_out < 1 || cfg->n_hidden < 1 || cfg->n_hidden_layers < 0 ||
@@ -380,7 +380,7 @@ int twmc_mlp_create(const twmc_mlp_config* cfg, int device, void** ctx_out) {
   if (e == hipSuccess) e = c->adam_v.ensure(off * 4);
   if (e != hipSuccess) {
     std::string m = std::string("twmc_mlp_create: ") + hipGetErrorString(e);
-    twmc_mlp_destroy(c);
+    twmc_mlp_destroy(reinterpret_cast<twmc_mlp_ctx*>(c));
     return mlp_fail(nullptr, TWMC_ERR_HIP, m);
   }
   // Glorot-uniform weights, zero biases, deterministic in cfg->seed
@@ -394,20 +394,20 @@ int twmc_mlp_create(const twmc_mlp_config* cfg, int device, void** ctx_out) {
       p[c->w_off[l] + i] = float((2.0 * u - 1.0) * a);
     }
   }
-  *ctx_out = c;
-  int rc = twmc_mlp_set_params(c, p.data());
+  *ctx_out = reinterpret_cast<twmc_mlp_ctx*>(c);
+  int rc = twmc_mlp_set_params(*ctx_out, p.data());
   if (rc != TWMC_OK) {
     std::string m = c->err;
-    twmc_mlp_destroy(c);
+    twmc_mlp_destroy(reinterpret_cast<twmc_mlp_ctx*>(c));
     *ctx_out = nullptr;
     return mlp_fail(nullptr, rc, m);
   }
   return TWMC_OK;
 }
 
-void twmc_mlp_destroy(void* ctx) {
+void twmc_mlp_destroy(twmc_mlp_ctx* ctx) {
   if (!ctx) return;
-  MlpCtx* c = static_cast<MlpCtx*>(ctx);
+  MlpCtx* c = reinterpret_cast<MlpCtx*>(ctx);
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   Buf* bufs[] = {&c->params, &c->grads, &c->adam_m, &c->adam_v, &c->dA, &c->dB, &c->data_x, &c->data_t,
@@ -419,20 +419,20 @@ void twmc_mlp_destroy(void* ctx) {
   delete c;
 }
 
-uint64_t twmc_mlp_n_params(const void* ctx) { return ctx ? static_cast<const MlpCtx*>(ctx)->n_params : 0; }
+uint64_t twmc_mlp_n_params(const twmc_mlp_ctx* ctx) { return ctx ? reinterpret_cast<const MlpCtx*>(ctx)->n_params : 0; }
 
-int twmc_mlp_get_params(void* ctx, float* out) {
+int twmc_mlp_get_params(twmc_mlp_ctx* ctx, float* out) {
   if (!ctx || !out) return mlp_fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  MlpCtx* c = static_cast<MlpCtx*>(ctx);
+  MlpCtx* c = reinterpret_cast<MlpCtx*>(ctx);
   MLPCHK(c, hipSetDevice(c->device));
   MLPCHK(c, hipStreamSynchronize(c->stream));
   MLPCHK(c, hipMemcpy(out, c->params.p, c->n_params * 4, hipMemcpyDeviceToHost));
   return TWMC_OK;
 }
 
-int twmc_mlp_set_params(void* ctx, const float* in) {
+int twmc_mlp_set_params(twmc_mlp_ctx* ctx, const float* in) {
   if (!ctx || !in) return mlp_fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  MlpCtx* c = static_cast<MlpCtx*>(ctx);
+  MlpCtx* c = reinterpret_cast<MlpCtx*>(ctx);
   MLPCHK(c, hipSetDevice(c->device));
   MLPCHK(c, hipStreamSynchronize(c->stream));
   MLPCHK(c, hipMemcpy(c->params.p, in, c->n_params * 4, hipMemcpyHostToDevice));
@@ -443,9 +443,9 @@ int twmc_mlp_set_params(void* ctx, const float* in) {
   return TWMC_OK;
 }
 
-int twmc_mlp_forward(void* ctx, const float* x, uint64_t rows, float* y) {
+int twmc_mlp_forward(twmc_mlp_ctx* ctx, const float* x, uint64_t rows, float* y) {
   if (!ctx || !x || !y) return mlp_fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  MlpCtx* c = static_cast<MlpCtx*>(ctx);
+  MlpCtx* c = reinterpret_cast<MlpCtx*>(ctx);
   if (rows == 0 || rows > (1u << 24)) return mlp_fail(c, TWMC_ERR_INVALID, "rows out of range");
   MLPCHK(c, hipSetDevice(c->device));
   int rc = ensure_rows(c, (int)rows);
@@ -459,10 +459,10 @@ int twmc_mlp_forward(void* ctx, const float* x, uint64_t rows, float* y) {
   return TWMC_OK;
 }
 
-int twmc_mlp_loss_grad(void* ctx, const float* x, const float* t, const float* w, uint64_t rows, double* loss,
+int twmc_mlp_loss_grad(twmc_mlp_ctx* ctx, const float* x, const float* t, const float* w, uint64_t rows, double* loss,
                        float* grad) {
   if (!ctx || !x || !t) return mlp_fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  MlpCtx* c = static_cast<MlpCtx*>(ctx);
+  MlpCtx* c = reinterpret_cast<MlpCtx*>(ctx);
   if (rows == 0 || rows > (1u << 24)) return mlp_fail(c, TWMC_ERR_INVALID, "rows out of range");
   MLPCHK(c, hipSetDevice(c->device));
   int rc = ensure_rows(c, (int)rows);
@@ -484,10 +484,10 @@ int twmc_mlp_loss_grad(void* ctx, const float* x, const float* t, const float* w
   return TWMC_OK;
 }
 
-int twmc_mlp_train(void* ctx, const float* x, const float* t, const float* w, uint64_t rows,
+int twmc_mlp_train(twmc_mlp_ctx* ctx, const float* x, const float* t, const float* w, uint64_t rows,
                    const twmc_mlp_train_opts* o, double* epoch_loss) {
   if (!ctx || !x || !t || !o) return mlp_fail(nullptr, TWMC_ERR_INVALID, "null argument");
-  MlpCtx* c = static_cast<MlpCtx*>(ctx);
+  MlpCtx* c = reinterpret_cast<MlpCtx*>(ctx);
   if (rows == 0 || rows > (1u << 24)) return mlp_fail(c, TWMC_ERR_INVALID, "rows out of range");
   if (o->epochs < 0 || o->batch < 1 || !(o->lr > 0.0f) || !(o->beta1 >= 0.0f && o->beta1 < 1.0f) ||
       !(o->beta2 >= 0.0f && o->beta2 < 1.0f) || !(o->eps > 0.0f))

@@ -10,9 +10,13 @@
 // frame capture takes 2D points one trajectory per item (its FFT runs along
 // both dimensions) and chains as stacked rows, transformed along the rows
 // only.  tb per lattice shape is chosen by twmc_sweep_autotune, which times
-// the candidates on the device.  The per-step arithmetic is the
-// LDS kernel's (same Stockham passes, K product, Philox increments, update),
-// so every trajectory of a sweep equals the same trajectory run alone.
+// the candidates on the device.  A step applies the hopping as the exact
+// nearest-neighbour form of the reference's FFT product (the x- and
+// y-neighbours read from the LDS copy of the state), then the same Philox
+// increments and Euler update as the single-system kernels, so every
+// trajectory of a sweep equals the same trajectory run alone (to FP32
+// rounding); the Stockham FFT runs only in the frame capture, for the
+// k-space bins.
 // The frame loop (counter reset + step kernel per frame) can be captured once
 // into a hipGraph and launched as one graph.
 #include <hip/hip_runtime.h>
@@ -635,14 +639,16 @@ void stack_release(StackPlan& sp) {
 
 extern "C" {
 
-const char* twmc_sweep_last_error(const void* ctx) {
-  return ctx ? static_cast<const SweepCtx*>(ctx)->err.c_str() : g_sweep_error.c_str();
+const char* twmc_sweep_last_error(const twmc_sweep_ctx* ctx) {
+  return ctx ? reinterpret_cast<const SweepCtx*>(ctx)->err.c_str() : g_sweep_error.c_str();
 }
 
 int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt, uint64_t frame_steps,
-                      uint64_t seed, int device, void** ctx_out) {
+                      uint64_t seed, int device, twmc_sweep_ctx** ctx_out) {
   if (!pts || !ctx_out || n_points == 0) return sfail(nullptr, TWMC_ERR_INVALID, "empty sweep");
   if (!(dt > 0.0) || frame_steps < 1) return sfail(nullptr, TWMC_ERR_INVALID, "dt and frame_steps must be positive");
+  // one launch advances one frame; its step count is an int kernel argument
+  if (frame_steps > (uint64_t(1) << 30)) return sfail(nullptr, TWMC_ERR_INVALID, "frame_steps above 2^30");
   *ctx_out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return sfail(nullptr, TWMC_ERR_HIP, "no HIP device available");
@@ -739,7 +745,7 @@ int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt,
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) {
     std::string m = std::string("sweep setup: ") + hipGetErrorString(e);
-    twmc_sweep_destroy(c);
+    twmc_sweep_destroy(reinterpret_cast<twmc_sweep_ctx*>(c));
     return sfail(nullptr, TWMC_ERR_HIP, m);
   }
   c->grid = std::max(1, per_cu) * cus;
@@ -747,16 +753,16 @@ int twmc_sweep_create(const twmc_sweep_point* pts, uint64_t n_points, double dt,
   int rc = build_items(c);
   if (rc) {
     std::string m = c->err;
-    twmc_sweep_destroy(c);
+    twmc_sweep_destroy(reinterpret_cast<twmc_sweep_ctx*>(c));
     return sfail(nullptr, rc, m);
   }
-  *ctx_out = c;
+  *ctx_out = reinterpret_cast<twmc_sweep_ctx*>(c);
   return TWMC_OK;
 }
 
-void twmc_sweep_destroy(void* ctx) {
+void twmc_sweep_destroy(twmc_sweep_ctx* ctx) {
   if (!ctx) return;
-  SweepCtx* c = static_cast<SweepCtx*>(ctx);
+  SweepCtx* c = reinterpret_cast<SweepCtx*>(ctx);
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* bufs[] = {c->d_points, c->d_items, c->d_items_cap, c->d_counter, c->d_tw, c->d_kc, c->d_beta, c->d_m2, c->d_m4, c->d_binit,
@@ -767,9 +773,9 @@ void twmc_sweep_destroy(void* ctx) {
   delete c;
 }
 
-int twmc_sweep_set_batching(void* ctx, const int32_t* tb) {
+int twmc_sweep_set_batching(twmc_sweep_ctx* ctx, const int32_t* tb) {
   if (!ctx || !tb) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
-  SweepCtx* c = static_cast<SweepCtx*>(ctx);
+  SweepCtx* c = reinterpret_cast<SweepCtx*>(ctx);
   for (size_t p = 0; p < c->pts.size(); ++p) {
     const SweepPointDev& P = c->dev_pts[p];
     const int v = tb[p];
@@ -780,19 +786,19 @@ int twmc_sweep_set_batching(void* ctx, const int32_t* tb) {
   return build_items(c);
 }
 
-int twmc_sweep_get_batching(void* ctx, int32_t* tb) {
+int twmc_sweep_get_batching(twmc_sweep_ctx* ctx, int32_t* tb) {
   if (!ctx || !tb) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
-  SweepCtx* c = static_cast<SweepCtx*>(ctx);
+  SweepCtx* c = reinterpret_cast<SweepCtx*>(ctx);
   for (size_t p = 0; p < c->tb.size(); ++p) tb[p] = c->tb[p];
   return TWMC_OK;
 }
 
 // Runs the sweep from the vacuum start; frames follow the reference loop with
 // store_initial.  use_graph captures the frame loop into one hipGraph.
-int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, double* traj_m2, double* traj_m4,
+int twmc_sweep_run(twmc_sweep_ctx* ctx, double t_start, double t_end, int32_t use_graph, double* traj_m2, double* traj_m4,
                    double* ksum, uint64_t* n_frames_out, double* kernel_ms) {
   if (!ctx) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
-  SweepCtx* c = static_cast<SweepCtx*>(ctx);
+  SweepCtx* c = reinterpret_cast<SweepCtx*>(ctx);
   SCHK(c, hipSetDevice(c->device));
   // frame schedule (TWMCBaseSolver.cpp:221-266), store_initial = true
   double dt_obs = c->dt * (double)c->frame_steps;
@@ -835,11 +841,21 @@ int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, d
   if (rc) return rc;
   SCHK(c, hipStreamSynchronize(c->stream));
 
-  hipEvent_t e0, e1;
-  SCHK(c, hipEventCreate(&e0));
-  SCHK(c, hipEventCreate(&e1));
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // events and graph of this run, released on every return path
+  struct RunResources {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~RunResources() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      if (e1) (void)hipEventDestroy(e1);
+      if (e0) (void)hipEventDestroy(e0);
+    }
+  } res;
+  SCHK(c, hipEventCreate(&res.e0));
+  SCHK(c, hipEventCreate(&res.e1));
+  hipEvent_t e0 = res.e0, e1 = res.e1;
   if (use_graph) SCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   else SCHK(c, hipEventRecord(e0, c->stream));
   for (size_t k = 0; k < cap.size(); ++k) {
@@ -861,19 +877,15 @@ int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, d
     }
   }
   if (use_graph) {
-    SCHK(c, hipStreamEndCapture(c->stream, &graph));
-    SCHK(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    SCHK(c, hipStreamEndCapture(c->stream, &res.graph));
+    SCHK(c, hipGraphInstantiate(&res.exec, res.graph, nullptr, nullptr, 0));
     SCHK(c, hipEventRecord(e0, c->stream));
-    SCHK(c, hipGraphLaunch(exec, c->stream));
+    SCHK(c, hipGraphLaunch(res.exec, c->stream));
   }
   SCHK(c, hipEventRecord(e1, c->stream));
   SCHK(c, hipStreamSynchronize(c->stream));
   float ms = 0.f;
   SCHK(c, hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
   c->stats.step_kernel_ms += ms;
   c->stats.step_launches += cap.size();
   c->stats.site_steps += 0;
@@ -890,9 +902,9 @@ int twmc_sweep_run(void* ctx, double t_start, double t_end, int32_t use_graph, d
   return TWMC_OK;
 }
 
-int twmc_sweep_mode_bins(void* ctx, uint64_t point, uint8_t* bins) {
+int twmc_sweep_mode_bins(twmc_sweep_ctx* ctx, uint64_t point, uint8_t* bins) {
   if (!ctx || !bins) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
-  SweepCtx* c = static_cast<SweepCtx*>(ctx);
+  SweepCtx* c = reinterpret_cast<SweepCtx*>(ctx);
   if (point >= c->pts.size()) return sfail(c, TWMC_ERR_INVALID, "point index out of range");
   const SweepPointDev& P = c->dev_pts[point];
   std::memcpy(bins, &c->bin_pool[P.bin_off], (size_t)P.nx * P.ny);
@@ -901,9 +913,9 @@ int twmc_sweep_mode_bins(void* ctx, uint64_t point, uint8_t* bins) {
 
 // Times each candidate stacking per lattice shape on a short run (steps steps
 // of every point of that shape) and keeps the fastest per site-step.
-int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out) {
+int twmc_sweep_autotune(twmc_sweep_ctx* ctx, int32_t steps, int32_t* tb_out) {
   if (!ctx) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
-  SweepCtx* c = static_cast<SweepCtx*>(ctx);
+  SweepCtx* c = reinterpret_cast<SweepCtx*>(ctx);
   if (steps < 1) steps = 20;
   std::map<std::pair<int, int>, std::vector<size_t>> by_shape;  // (rows, row length) -> points
   for (size_t p = 0; p < c->pts.size(); ++p) by_shape[{c->dev_pts[p].nx, c->dev_pts[p].ny}].push_back(p);
@@ -922,14 +934,14 @@ int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out) {
       // time only this shape: a context holding just these points
       std::vector<twmc_sweep_point> sub;
       for (size_t p : kv.second) sub.push_back(c->pts[p]);
-      void* tctx = nullptr;
+      twmc_sweep_ctx* tctx = nullptr;
       int rc = twmc_sweep_create(sub.data(), sub.size(), c->dt, (uint64_t)steps, c->seed, c->device, &tctx);
       if (rc) return sfail(c, rc, twmc_sweep_last_error(nullptr));
       std::vector<int32_t> subtb(sub.size(), cand);
       rc = twmc_sweep_set_batching(tctx, subtb.data());
       double ms = 0.0;
       if (!rc) rc = twmc_sweep_run(tctx, 0.0, c->dt * steps, 0, nullptr, nullptr, nullptr, nullptr, &ms);
-      uint64_t ss = static_cast<SweepCtx*>(tctx)->stats.site_steps;
+      uint64_t ss = reinterpret_cast<SweepCtx*>(tctx)->stats.site_steps;
       twmc_sweep_destroy(tctx);
       if (rc) return sfail(c, rc, "autotune run failed");
       double rate = ms > 0 ? double(ss) / ms : 0.0;
@@ -946,9 +958,9 @@ int twmc_sweep_autotune(void* ctx, int32_t steps, int32_t* tb_out) {
   return TWMC_OK;
 }
 
-int twmc_sweep_query(void* ctx, twmc_stats* st) {
+int twmc_sweep_query(twmc_sweep_ctx* ctx, twmc_stats* st) {
   if (!ctx || !st) return sfail(nullptr, TWMC_ERR_INVALID, "null argument");
-  *st = static_cast<SweepCtx*>(ctx)->stats;
+  *st = reinterpret_cast<SweepCtx*>(ctx)->stats;
   return TWMC_OK;
 }
 

@@ -1,5 +1,6 @@
 #include "hip_solver.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <iostream>
@@ -32,7 +33,7 @@ void to_matrix(const NoisyMatrix& m, twmc_matrix& out, std::vector<std::vector<d
     std::cerr << "warning: Lorentzian/Poissonian disorder is not generated (as in the reference)" << std::endl;
 }
 
-void check(int rc, void* ctx, const char* what) {
+void check(int rc, twmc_ctx* ctx, const char* what) {
   if (rc != TWMC_OK) {
     const char* e = twmc_last_error(ctx);
     throw std::runtime_error(std::string(what) + ": " + (e ? e : "") + " (code " + std::to_string(rc) + ")");
@@ -77,7 +78,9 @@ std::vector<double> LiebCoupling(size_t ny, double J_AB, double J_BC, bool pbc, 
 TWMCHipSolver::TWMCHipSolver() { nTasksToRequest = 4096; }
 
 TWMCHipSolver::~TWMCHipSolver() {
+  std::lock_guard<std::mutex> lk(ctx_mu_);
   if (ctx_) twmc_destroy(ctx_);
+  ctx_ = nullptr;
 }
 
 void TWMCHipSolver::SuggestBatchSize() {
@@ -106,8 +109,9 @@ void TWMCLiebHipSolver::FillLattice(const TWMCSystemData& d, twmc_params& p,
 // TWMCBaseSolver::Setup (TWMCBaseSolver.cpp:24-92): one context per system
 void TWMCHipSolver::Setup() {
   const TWMCSystemData& d = *lastSharedSystemData;
-  if (ctx_) {
-    twmc_destroy(ctx_);
+  {
+    std::lock_guard<std::mutex> lk(ctx_mu_);
+    if (ctx_) twmc_destroy(ctx_);
     ctx_ = nullptr;
   }
   twmc_params p{};
@@ -133,7 +137,12 @@ void TWMCHipSolver::Setup() {
     p.F_times = ft_times.data();
     p.F_table = ft_values.data();
   }
-  check(twmc_create(&p, gpu_, &ctx_), nullptr, "twmc_create");
+  twmc_ctx* created = nullptr;
+  check(twmc_create(&p, gpu_, &created), nullptr, "twmc_create");
+  {
+    std::lock_guard<std::mutex> lk(ctx_mu_);
+    ctx_ = created;
+  }
   nTasksToRequest = d.gpu_batch;
 }
 
@@ -147,7 +156,8 @@ std::vector<std::unique_ptr<TaskResults>> TWMCHipSolver::Compute(
     if (!t) throw std::runtime_error("TWMCHipSolver received a task that is not a TWMCTaskData");
     groups[{t->systemData.get(), t->t_start, t->t_end, (int)t->initialCondition, t->storeInitialState}].push_back(k);
   }
-  total_ += tasks.size();
+  call_total_ = tasks.size();
+  call_done_ = 0;
   for (auto& g : groups) {
     const std::vector<size_t>& idx = g.second;
     auto* first = static_cast<TWMCTaskData*>(tasks[idx[0]].get());
@@ -182,8 +192,13 @@ std::vector<std::unique_ptr<TaskResults>> TWMCHipSolver::Compute(
     if (first->initialCondition != TWMCTaskData::ReadFromPreviousData) desc.step0 = 0;
     desc.t_start = first->t_start;
     desc.t_end = first->t_end;
+    // store = traj writes the reference's datasets only (TWMCResults.cpp:26-38:
+    // traj and the disorder realisations); observables and both add the
+    // per-trajectory site means mean_n2 / mean_n4
     const bool want_traj = d.store != "observables";
+    const bool want_means = d.store != "traj";
     desc.outputs = (uint32_t)TWMC_OUT_OBS | (want_traj ? (uint32_t)TWMC_OUT_TRAJ : 0u);
+    group_size_ = ids.size();
     check(twmc_batch_begin(ctx_, &desc), ctx_, "twmc_batch_begin");
     twmc_batch_info info{};
     check(twmc_batch_info_get(ctx_, &info), ctx_, "twmc_batch_info_get");
@@ -212,8 +227,10 @@ std::vector<std::unique_ptr<TaskResults>> TWMCHipSolver::Compute(
       std::vector<size_t> dims = {d.nx, d.ny};
       if (d.cellSz != 1) dims.push_back(d.cellSz);
       if (want_traj) res->AddDataset(kDatasetTraj, kFormatComplex128, nf, dims, &traj[2 * n * nf * j], nf * n * 16);
-      res->AddDataset(kDatasetM2, kFormatFloat64, nf, {}, &m2[nf * j], nf * 8);
-      res->AddDataset(kDatasetM4, kFormatFloat64, nf, {}, &m4[nf * j], nf * 8);
+      if (want_means) {
+        res->AddDataset(kDatasetM2, kFormatFloat64, nf, {}, &m2[nf * j], nf * 8);
+        res->AddDataset(kDatasetM4, kFormatFloat64, nf, {}, &m4[nf * j], nf * 8);
+      }
       // disorder realisations, regenerated on the host from the same Philox draws
       auto add_realisation = [&](const NoisyMatrix& m, const char* name, uint32_t tag) {
         if (!m.HasNoise()) return;
@@ -226,14 +243,25 @@ std::vector<std::unique_ptr<TaskResults>> TWMCHipSolver::Compute(
       res->extraData = {t->t_start, info.t_final};
       all[idx[j]] = std::move(res);
     }
-    done_ += B;
+    call_done_ += B;
+    group_size_ = 0;
   }
   return all;
 }
 
+// Fraction of the current Compute call done: finished groups plus the
+// device's progress through the group it runs now (twmc_progress), the
+// counterpart of TWMCBaseSolver.cpp:293-295 for a batch.
 float TWMCHipSolver::ApproximateComputationProgress() const {
-  uint64_t tot = total_.load();
-  return tot ? float(double(done_.load()) / double(tot)) : 0.f;
+  const uint64_t tot = call_total_.load();
+  if (tot == 0) return 0.f;
+  double done = double(call_done_.load());
+  const uint64_t g = group_size_.load();
+  if (g) {
+    std::lock_guard<std::mutex> lk(ctx_mu_);
+    if (ctx_) done += double(g) * double(twmc_progress(ctx_));
+  }
+  return float(std::min(1.0, done / double(tot)));
 }
 
 namespace {

@@ -13,6 +13,7 @@
 #define PINCO_HIP_SOLVER_HPP
 
 #include <atomic>
+#include <mutex>
 #include <vector>
 
 #include "../../include/twmc_hip.h"
@@ -35,9 +36,14 @@ class TWMCHipSolver : public SolverGPU {
   // fills the lattice-specific part of the parameters; storage keeps arrays alive
   virtual void FillLattice(const TWMCSystemData& d, twmc_params& p, std::vector<std::vector<double>>& storage);
   const TWMCSystemData* lastSharedSystemData = nullptr;
-  void* ctx_ = nullptr;
+  // the context is replaced only under ctx_mu_, so that another thread's
+  // ApproximateComputationProgress never reads a destroyed one
+  twmc_ctx* ctx_ = nullptr;
+  mutable std::mutex ctx_mu_;
   int gpu_ = 0;
-  std::atomic<uint64_t> done_{0}, total_{0};
+  // progress of the current Compute call: its trajectories, those finished,
+  // and the size of the group on the device now
+  std::atomic<uint64_t> call_total_{0}, call_done_{0}, group_size_{0};
 };
 
 class TWMCLiebHipSolver : public TWMCHipSolver {

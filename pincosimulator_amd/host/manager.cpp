@@ -1,12 +1,11 @@
 #include "manager.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
-#include <condition_variable>
-#include <deque>
 #include <iostream>
-#include <mutex>
-#include <thread>
+
+#include "task_processor.hpp"
 
 namespace pinco {
 
@@ -30,8 +29,8 @@ TWMCManager::TWMCManager(const Settings& s) : settings_(s) {
 void TWMCManager::Setup() {
   store_ = std::make_unique<PincoFormatDataStore>(settings_.GetOutputFolder(),
                                                   settings_.getDouble("chunk_size", 1.5));
-  solver_ = SolverGPUFactory::makeOrThrow(solverName_);
-  solver_->SetupGPU((size_t)settings_.getInt("GPU_id", 0));
+  // the solver name must be registered before any worker starts
+  (void)SolverGPUFactory::makeOrThrow(solverName_);
   store_->SaveFile("_t.dat", sys_->GetStoredTimes());
   if (sys_->F->HasTimeDependence()) store_->SaveFile("_F_t.dat", sys_->GetStoredVariableEvolution(*sys_->F));
 }
@@ -76,68 +75,52 @@ std::vector<std::unique_ptr<TaskData>> TWMCManager::DispatchTasks() {
   return tasks;
 }
 
+// ManagerLoop (TWMCManager.cpp:113-131) with the reference's processor
+// (ThreadedTaskProcessor.cpp:66-101): `gpu_workers` worker threads on GPU
+// `GPU_id`, each owning one GPU solver (its own context and stream, so one
+// worker's host-side work -- task grouping, result assembly, archive
+// formatting -- overlaps another's kernels), a saver thread writing the
+// archive and a reporter thread printing progress every
+// `progress_interval` seconds.
 size_t TWMCManager::ManagerLoop() {
   std::vector<std::unique_ptr<TaskData>> tasks = DispatchTasks();
   std::cout << "Enqueued " << tasks.size() << " tasks. " << std::endl;
-  // saver thread: drains finished batches into the archive while the GPU
-  // computes the next batch (ResultsSaver, src/Base/Modules/ResultsSaver.cpp:34-52)
-  std::mutex mu;
-  std::condition_variable cv;
-  std::deque<std::vector<std::unique_ptr<TaskResults>>> queue;
-  bool finished = false;
-  size_t saved = 0;
-  std::thread saver([&] {
-    for (;;) {
-      std::vector<std::unique_ptr<TaskResults>> batch;
-      {
-        std::unique_lock<std::mutex> lk(mu);
-        cv.wait(lk, [&] { return finished || !queue.empty(); });
-        if (queue.empty() && finished) return;
-        batch = std::move(queue.front());
-        queue.pop_front();
-      }
-      for (auto& r : batch) {
-        store_->StoreData(*r);
-        ++saved;
-      }
-    }
-  });
-  const size_t per_call = std::max<size_t>(1, sys_->gpu_batch);
-  auto t0 = std::chrono::steady_clock::now();
-  size_t done = 0;
+  const size_t n_workers = (size_t)std::max<int64_t>(1, settings_.getInt("gpu_workers", 1));
+  const size_t gpu = (size_t)settings_.getInt("GPU_id", 0);
+  const size_t batch = std::max<size_t>(1, sys_->gpu_batch);
+  const std::string name = solverName_;
+  ThreadedTaskProcessor proc(
+      [name, gpu, batch](size_t) {
+        std::unique_ptr<SolverGPU> s = SolverGPUFactory::makeOrThrow(name);
+        s->SetupGPU(gpu);
+        s->nTasksToRequest = batch;
+        return std::unique_ptr<Solver>(s.release());
+      },
+      n_workers);
+  PincoFormatDataStore* store = store_.get();
+  ResultsSaver saver([store](const TaskResults& r) { store->StoreData(r); });
+  proc.SetConsumer(&saver);
+  proc.Setup();
+  proc.EnqueueTasks(std::move(tasks));
+  proc.AllProducersHaveBeenTerminated();
+  const double every = settings_.getDouble("progress_interval", 10.0);
+  ProgressReporter reporter(proc, &saver,
+                            std::chrono::milliseconds((int64_t)std::max(1.0, every * 1000.0)), std::cout);
   try {
-    for (size_t k = 0; k < tasks.size(); k += per_call) {
-      size_t end = std::min(tasks.size(), k + per_call);
-      std::vector<std::unique_ptr<TaskData>> batch;
-      for (size_t j = k; j < end; ++j) batch.push_back(std::move(tasks[j]));
-      auto results = solver_->Compute(batch);
-      done += results.size();
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        queue.push_back(std::move(results));
-      }
-      cv.notify_one();
-      double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      std::cout << "progress: " << done << "/" << tasks.size() << " trajectories, " << el << " s" << std::endl;
-    }
+    proc.Wait();
   } catch (...) {
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      finished = true;
+    reporter.Stop();
+    try {
+      saver.Finish();  // keep what finished before the failure
+    } catch (...) {
     }
-    cv.notify_one();
-    saver.join();
     throw;
   }
-  {
-    std::lock_guard<std::mutex> lk(mu);
-    finished = true;
-  }
-  cv.notify_one();
-  saver.join();
+  reporter.Stop();
+  saver.Finish();
   SaveObservables();
   std::cout << "Exited the ManagerLoop()" << std::endl;
-  return saved;
+  return saver.SavedItems();
 }
 
 void TWMCManager::SaveObservables() const {

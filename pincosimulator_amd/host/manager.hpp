@@ -5,8 +5,10 @@
 // lattice == "lieb" -> TWMCLieb; PBC with a constant F -> TWMCBase; anything
 // else has no eligible solver.  One task per trajectory id; ids already in the
 // output archive are skipped, and stored trajectories that end before t_end
-// get a continuation task (ReadFromPreviousData).  Results pass through a
-// saver thread into the PincoFormat archive (the reference's ResultsSaver).
+// get a continuation task (ReadFromPreviousData).  The tasks go through a
+// ThreadedTaskProcessor (worker threads, one GPU solver each); results pass
+// through a saver thread into the PincoFormat archive (the reference's
+// ResultsSaver) and a reporter thread prints the progress.
 #ifndef PINCO_MANAGER_HPP
 #define PINCO_MANAGER_HPP
 
@@ -43,7 +45,6 @@ class TWMCManager {
   std::shared_ptr<TWMCSystemData> sys_;
   std::string solverName_;
   std::unique_ptr<PincoFormatDataStore> store_;
-  std::unique_ptr<SolverGPU> solver_;
 };
 
 }  // namespace pinco

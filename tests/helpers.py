@@ -50,3 +50,11 @@ def oracle_traj(spec, ids, **kw):
 
 def rel_err(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+# Parity bars of the FP32 paths, each about ten times the largest error measured
+# on MI355X for that path (FP32 rounding only: the FP32 and FP64 runs share the
+# Philox stream), so that a systematic FP32 regression of ~1e-5 relative fails.
+FP32_BAR = 2e-5       # FP32 kernels vs the FP64 oracle, per trajectory, relative to max |beta| (<= 1000 steps)
+MFMA_BAR = 5e-5       # BF16 hi/lo x3 MFMA hopping (FP32 state) vs the FP64 oracle, per trajectory
+OBS_SIGMA_BAR = 0.05  # FP32 vs FP64 ensemble observables, in standard errors of the FP64 ensemble

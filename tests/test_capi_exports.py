@@ -65,3 +65,21 @@ def test_kernel_ids_match_the_header():
     for name, value in ids.items():
         assert _lib.KERNEL_NAMES[value] == name
         assert engine._KERNELS[name] == value
+
+
+def _compile(snippet):
+    src = '#include "twmc_hip.h"\n' + snippet
+    return subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-x", "c++", "-I", os.path.join(ROOT, "include"),
+                           "-"], input=src, capture_output=True, text=True)
+
+
+def test_handles_are_distinct_types():
+    """An engine, sweep or network handle passed where another kind is expected does not compile."""
+    ok = _compile("void f(twmc_ctx* c, twmc_sweep_ctx* s, twmc_mlp_ctx* m) {"
+                  " twmc_batch_sync(c); twmc_sweep_destroy(s); twmc_mlp_destroy(m); }")
+    assert ok.returncode == 0, ok.stderr
+    for bad in ("void f(twmc_sweep_ctx* s) { twmc_run(s, 0, 0); }",
+                "void f(twmc_mlp_ctx* m) { twmc_batch_sync(m); }",
+                "void f(twmc_ctx* c) { twmc_sweep_destroy(c); }"):
+        r = _compile(bad)
+        assert r.returncode != 0 and "twmc_" in r.stderr, bad

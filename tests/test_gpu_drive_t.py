@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import rel_err, system
+from helpers import MFMA_BAR, rel_err, system
 from pincosimulator_amd import Engine
 from pincosimulator_amd.pinco_format import Archive
 
@@ -72,7 +72,7 @@ def test_drive_table_mfma_matches_oracle():
     eng.close()
     assert res.kernel == "mfma"
     ref = oracle_frames(spec, ids[:8])
-    assert rel_err(res.traj[:8], ref) < 2e-4
+    assert rel_err(res.traj[:8], ref) < MFMA_BAR
 
 
 def test_drive_table_mfma_overlapped_step_matches_oracle():
@@ -84,7 +84,7 @@ def test_drive_table_mfma_overlapped_step_matches_oracle():
     res = eng.run(ids, traj=True)
     eng.close()
     assert res.kernel == "mfma"
-    assert rel_err(res.traj, oracle_frames(spec, ids)) < 2e-4
+    assert rel_err(res.traj, oracle_frames(spec, ids)) < MFMA_BAR
 
 
 def test_drive_table_is_not_sent_to_the_fused_kernel():

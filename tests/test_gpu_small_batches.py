@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import rel_err, system
+from helpers import FP32_BAR, MFMA_BAR, rel_err, system
 from pincosimulator_amd import Engine, System
 from pincosimulator_amd.observables import from_trajectories
 from test_gpu_mfma import c3_spec
@@ -36,7 +36,7 @@ def test_small_batch(kernel, spec, prec, ids):
     lat = oracle.LAT_DENSE if spec.get("lattice") == "matrix" else oracle.LAT_FFT
     p = dict(spec, lattice=lat, E=spec["omega"]) if lat == oracle.LAT_DENSE else dict(spec, lattice=lat)
     err = max(rel_err(res.traj[j], oracle.run_trajectory(p, int(t))[0]) for j, t in enumerate(ids))
-    tol = {64: 1e-9}.get(prec, 1e-3 if kernel == "mfma" else 2e-4)
+    tol = {64: 1e-9}.get(prec, MFMA_BAR if kernel == "mfma" else FP32_BAR)
     assert err < tol, err
     shape = (spec["nx"], spec["ny"])
     ref = from_trajectories(res.traj, shape)

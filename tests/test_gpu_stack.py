@@ -2,13 +2,13 @@
 run as one point of the main API, trajectories stacked per work item, the
 batch's own trajectory ids): per trajectory against the FP64 oracle for
 chains and 2D shapes with scattered, unsorted ids, against the LDS kernel,
-ensemble sums against the dump, and the AUTO selection.  FP32 bar: 2e-4 of
+ensemble sums against the dump, and the AUTO selection.  FP32 bar: FP32_BAR of
 max |beta| per trajectory."""
 import numpy as np
 import pytest
 
 import oracle
-from helpers import rel_err, system
+from helpers import FP32_BAR, rel_err, system
 from pincosimulator_amd import Engine
 from pincosimulator_amd._lib import TWMCError
 from pincosimulator_amd.observables import from_trajectories
@@ -36,7 +36,7 @@ def test_stack_vs_fp64_oracle(nx, ny):
     err = max(rel_err(res.traj[j], oracle.run_trajectory(dict(sp, lattice=oracle.LAT_FFT), int(t))[0])
               for j, t in enumerate(IDS))
     print(f"  max relative deviation {err:.2e}")
-    assert err < 2e-4
+    assert err < FP32_BAR
 
 
 def test_stack_vs_lds_kernel_and_sums():

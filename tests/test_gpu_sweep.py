@@ -5,6 +5,7 @@ speed."""
 import numpy as np
 import pytest
 
+from helpers import OBS_SIGMA_BAR
 from pincosimulator_amd import Engine, Sweep, System
 
 pytestmark = pytest.mark.gpu
@@ -145,4 +146,42 @@ def test_sweep_full_length_error_vs_fp64():
         dn = np.max(np.abs(n32 - n64)[1:] / se_n[1:])
         dg = np.max(np.abs(g32 - g64)[1:] / se_g[1:])
         print(f"point {k} ({p['nx']}x{p['ny']}): <n> {dn:.2e}, g2(0) {dg:.2e} standard errors over 60 frames")
-        assert dn < 3 and dg < 3
+        assert dn < OBS_SIGMA_BAR and dg < OBS_SIGMA_BAR
+
+
+def test_config_sweep_512_points_in_one_launch():
+    """The sweep configuration's own 512-point list (32 x 16 grid of F and
+    detuning, the nine shapes round-robin: chains of 32, 64, 128, 256 and 512
+    sites, squares of side 16, 32, 48 and 64; 128 trajectories each) in one
+    persistent launch per frame, stacking autotuned, frame loop graph-captured,
+    2 frames of 500 steps.  One point of each shape, spread over the grid,
+    against the same system run alone (per-trajectory site means), and its
+    radial k-space bins and mean field against the single run's per-mode sums."""
+    from pincosimulator_amd.workloads import SWEEP_SHAPES, sweep512_points
+    pts = sweep512_points()
+    t_end = 2.0
+    sw = Sweep(pts, 0.002, 500, SEED)
+    tb = sw.autotune(steps=20)
+    print("autotuned stacking per shape:", {SWEEP_SHAPES[s]: tb[s] for s in range(9)})
+    out = sw.run(t_end, graph=True)
+    assert out["n_frames"] == 3
+    for s in range(9):
+        k = s + 9 * ((7 * s + 3) % 56)  # the s-th shape, at varying grid positions
+        p = pts[k]
+        assert (p["nx"], p["ny"]) == SWEEP_SHAPES[s]
+        sysp = System(nx=p["nx"], ny=p["ny"], dt=0.002, gamma=1.0, J=1.0, omega=p["omega"], U=0.05, F=p["F"],
+                      beta_init=0.0, beta_sigma=0.5, frame_steps=500, seed=SEED, t_start=0.0, t_end=t_end)
+        eng = Engine(sysp, precision=32, kernel="generic")
+        r = eng.run(np.arange(128) + p["traj_id0"])
+        eng.close()
+        d2 = np.max(np.abs(out["m2"][k] - r.traj_m2)) / np.max(np.abs(r.traj_m2))
+        d4 = np.max(np.abs(out["m4"][k] - r.traj_m4)) / np.max(np.abs(r.traj_m4))
+        bins = sw.mode_bins(k).reshape(-1)
+        ref = np.stack([r.sum_nk[:, bins == b].sum(axis=1) for b in range(32)], axis=1)
+        dk = np.max(np.abs(out["ksum"][k, :, :32] - ref)) / np.max(np.abs(ref))
+        mf = r.sum_beta.sum(axis=1)
+        dm = np.max(np.abs(out["ksum"][k, :, 32] + 1j * out["ksum"][k, :, 33] - mf)) / max(np.max(np.abs(mf)), 1.0)
+        print(f"point {k} {p['nx']}x{p['ny']} F={p['F']:.2f} D={p['omega']:.2f}: m2 {d2:.1e} m4 {d4:.1e} "
+              f"k-bins {dk:.1e} mean field {dm:.1e}")
+        assert d2 < 1e-5 and d4 < 1e-5 and dk < 1e-5 and dm < 1e-5
+    sw.close()

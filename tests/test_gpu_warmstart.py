@@ -41,12 +41,19 @@ def test_training_on_steady_states_generalises():
 
 
 def test_warm_start_reaches_steady_state_sooner():
+    """The learned warm start against the cold vacuum start on four held-out
+    points spread over the 64, at the configuration's 16384 trajectories: each
+    warm start is accepted (its late <n> and g2(0) within 3 sigma of the cold
+    run's steady state) and reaches the steady state at least 3x sooner."""
+    from pincosimulator_amd.workloads import WARMSTART_POINT_INDICES
     net = ws.WarmStart.load()
     F, D = ws.held_out_points(64)
-    for k in range(2):
+    for k in WARMSTART_POINT_INDICES:
         print(f"warm vs cold, held-out point {k}", flush=True)
-        r = ws.evaluate_point(float(F[k]), float(D[k]), net, n_traj=4096, log=lambda m: print(m, flush=True))
+        r = ws.evaluate_point(float(F[k]), float(D[k]), net, n_traj=16384, log=lambda m: print(m, flush=True))
+        print(f"  cold {r['cold_steps']} steps, warm {r['warm_steps']} steps, accepted {r['accepted']}")
         assert r["accepted"], r
         assert r["cold_steps"] is not None and r["warm_steps"] is not None
-        assert 2 * r["warm_steps"] <= r["cold_steps"], r
+        # a warm run already steady at its first frame counts as one frame (100 steps)
+        assert r["cold_steps"] >= 3 * max(r["warm_steps"], 100), r
     net.close()

@@ -236,3 +236,120 @@ def test_lieb_lattice_equals_dense_lieb_coupling():
     C = oracle.lieb_coupling(cells, 1.5, 0.7, True)
     b, _, _ = oracle.run_trajectory(dict(base, ny=3 * cells, lattice=oracle.LAT_DENSE, coupling=C), 2)
     np.testing.assert_array_equal(a, b)
+
+
+# ---------------------------------------------------------------------------
+# The matrix-hopping branch (TWMCLiebSolver.cpp:44-45 delta = -E + detuning,
+# :259-263 a = dt[(rsl + iU(|b|^2-1)) b - i (b.C) + iF]) pinned against
+# consequences of the mathematics, not against a restatement of the formula.
+
+def _ring(n):
+    """Adjacency of a periodic ring of n >= 3 sites."""
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, (i + 1) % n] = A[(i + 1) % n, i] = 1.0
+    return A
+
+
+def _torus(nx, ny):
+    """Adjacency of an nx x ny periodic square lattice (row-major sites), nx, ny >= 3."""
+    n = nx * ny
+    A = np.zeros((n, n))
+    for x in range(nx):
+        for y in range(ny):
+            s = x * ny + y
+            for t in (((x + 1) % nx) * ny + y, x * ny + (y + 1) % ny):
+                A[s, t] += 1.0
+                A[t, s] += 1.0
+    return A
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 12), (6, 5), (4, 8)])
+@pytest.mark.parametrize("detuning,E", [(0.5, 0.0), (0.9, 0.4), (-0.3, -1.1)])
+def test_dense_nearest_neighbour_coupling_equals_fft_hopping(nx, ny, detuning, E):
+    """The FFT hopping IDFT(K . DFT b)/N with K = -2iJ([nx!=1]cos kx + [ny!=1]cos ky) is, term by term,
+    -iJ times the sum over the periodic nearest neighbours.  So the matrix branch with C = J x the ring
+    (torus) adjacency and on-site detuning - E must reproduce the FFT branch with omega = detuning - E, on
+    the same Philox increments, over a few hundred noisy, nonlinear, driven steps.  The FFT branch's
+    sign and normalisation are pinned by the plane-wave eigenvalue test above; a flipped delta sign
+    (E - detuning), a flipped hopping sign (+i b.C) or a dropped term fails here."""
+    J = 0.8
+    A = _ring(ny) if nx == 1 else _torus(nx, ny)
+    common = dict(nx=nx, ny=ny, dt=0.005, gamma=1.0, J=J, U=0.15, F=0.9 - 0.3j, beta_init=0.2 + 0.1j,
+                  beta_sigma=0.5, frame_steps=60, seed=4242, t_start=0.0, t_end=1.5)
+    fft_p = dict(common, lattice=oracle.LAT_FFT, omega=detuning - E)
+    dense_p = dict(common, lattice=oracle.LAT_DENSE, coupling=J * A, detuning=detuning, E=E)
+    for traj in (0, 5):
+        a, af, _ = oracle.run_trajectory(fft_p, traj)
+        b, bf, _ = oracle.run_trajectory(dense_p, traj)
+        assert a.shape == b.shape and a.shape[0] == 6
+        scale = np.max(np.abs(a))
+        assert np.max(np.abs(a - b)) < 1e-13 * max(scale, 1.0) * 10, np.max(np.abs(a - b))
+        assert np.max(np.abs(af - bf)) < 1e-12
+
+
+def test_dense_site_resolved_detuning_equals_fft_site_resolved_omega():
+    """Site-resolved E (a base array) enters as delta_i = detuning - E_i: the FFT branch with the
+    site-resolved omega_i = detuning - E_i must agree."""
+    nx, ny, J = 1, 10, 1.0
+    rng = np.random.default_rng(3)
+    E = rng.normal(size=nx * ny)
+    common = dict(nx=nx, ny=ny, dt=0.004, gamma=1.0, J=J, U=0.1, F=1.0, beta_init=0.0, beta_sigma=0.5,
+                  frame_steps=50, seed=77, t_start=0.0, t_end=1.0)
+    a, _, _ = oracle.run_trajectory(dict(common, lattice=oracle.LAT_FFT, omega=0.6 - E), 1)
+    b, _, _ = oracle.run_trajectory(dict(common, lattice=oracle.LAT_DENSE, coupling=J * _ring(ny),
+                                         detuning=0.6, E=E), 1)
+    assert np.max(np.abs(a - b)) < 1e-12
+
+
+@pytest.mark.parametrize("cells,pbc", [(5, False), (6, True)])
+def test_lieb_eigenvector_evolves_geometrically(cells, pbc):
+    """U = gamma = F = 0 (so no noise): for an eigenvector v of the Lieb coupling, C v = lambda v, the
+    Euler map is b -> b + dt(-i delta b - i b.C) = (1 - i dt (delta + lambda)) b, so
+    b(n) = (1 - i dt (detuning - E + lambda))^n v exactly.  lambda and v come from numpy.linalg.eigh of
+    the coupling the solver builds (itself pinned by the enumeration test)."""
+    J_AB, J_BC, detuning, E, dt, fs = 1.3, 0.6, 0.45, 0.2, 0.01, 25
+    C = oracle.lieb_coupling(cells, J_AB, J_BC, pbc)
+    lam, V = np.linalg.eigh(C)
+    for k in (0, len(lam) // 2, len(lam) - 1):
+        v = V[:, k].astype(np.complex128) * (0.7 + 0.2j)
+        p = dict(nx=1, ny=cells, lattice=oracle.LAT_LIEB, pbc=int(pbc), dt=dt, gamma=0.0, J_AB=J_AB, J_BC=J_BC,
+                 detuning=detuning, E=E, U=0.0, F=0.0, beta_init=v, beta_sigma=0.0, frame_steps=fs, seed=1,
+                 t_start=0.0, t_end=2.0)
+        frames, _, _ = oracle.run_trajectory(p, 0, init_mode=oracle.INIT_FIXEDPOINT)
+        g = 1 - 1j * dt * (detuning - E + lam[k])
+        for f in range(frames.shape[0]):
+            np.testing.assert_allclose(frames[f], g ** (fs * f) * v, rtol=0, atol=1e-12)
+
+
+def test_dense_left_eigenvector_pins_row_vector_product():
+    """The reference multiplies the row vector b by C from the right (b.C, TWMCLiebSolver.cpp:262).
+    For an asymmetric banded C the left eigenvector w (w.C = mu w) evolves geometrically under
+    U = gamma = F = 0; the right eigenvector (C u = mu u) does not, so C.b in place of b.C fails."""
+    n, dt, fs, detuning, E = 9, 0.01, 20, 0.35, -0.15
+    C = np.zeros((n, n))
+    for i in range(n - 1):
+        C[i, i + 1], C[i + 1, i] = 1.4, 0.5       # products > 0: real spectrum
+    for i in range(n - 2):
+        C[i, i + 2], C[i + 2, i] = 0.3, 0.12
+    mu, W = np.linalg.eig(C.T)                   # C^T w = mu w  <=>  w.C = mu w
+    order = np.argsort(mu.real)
+    mu_r, U_r = np.linalg.eig(C)
+
+    def run(v):
+        p = dict(nx=1, ny=n, lattice=oracle.LAT_DENSE, coupling=C, dt=dt, gamma=0.0, J=0.0, detuning=detuning,
+                 E=E, U=0.0, F=0.0, beta_init=v, beta_sigma=0.0, frame_steps=fs, seed=2, t_start=0.0, t_end=1.0)
+        return oracle.run_trajectory(p, 0, init_mode=oracle.INIT_FIXEDPOINT)[0]
+
+    for k in (order[0], order[-1]):
+        w = W[:, k].astype(np.complex128)
+        frames = run(w)
+        g = 1 - 1j * dt * (detuning - E + mu[k])
+        for f in range(frames.shape[0]):
+            np.testing.assert_allclose(frames[f], g ** (fs * f) * w, rtol=0, atol=1e-12)
+    # discrimination: a right eigenvector is not transported geometrically by b.C
+    k = int(np.argmax(mu_r.real))
+    u = U_r[:, k].astype(np.complex128)
+    frames = run(u)
+    g = 1 - 1j * dt * (detuning - E + mu_r[k])
+    assert np.max(np.abs(frames[-1] - g ** (fs * (frames.shape[0] - 1)) * u)) > 1e-3
