@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 METRIC = "trajectory·site-steps/s (2D 64×64, 1 GPU); time-to-steady-state plain vs warm"
 PEAK_FP32 = 157.3e12           # MI355X FP32 vector peak (FLOP/s)
 PEAK_BF16 = 2.5e15             # MI355X dense BF16 MFMA peak (FLOP/s)
+PEAK_FP64 = 78.6e12            # MI355X FP64 vector peak (FLOP/s)
 
 # square64: the headline workload (default).  lieb384: the disordered Lieb
 # chain with matrix hopping as the dense MFMA product (the banded kernel timed
@@ -264,9 +265,6 @@ def time_frames(eng, n_traj, W, K):
     return eng, info, st, t1 - t0
 
 
-PEAK_FP64 = 78.6e12            # MI355X FP64 vector peak (FLOP/s)
-
-
 def measure_engine(base, n_traj, fsteps, kernel, precision, W, K):
     """W untimed and K timed frames of one Engine workload; returns
     (site-steps/s over the wall clock, average step-kernel launch in ms, info)."""
@@ -348,6 +346,8 @@ def main():
     ap.add_argument("--no-warmstart", action="store_true", help="square64: skip the warm-start leg")
     ap.add_argument("--no-secondary", action="store_true",
                     help="square64: skip the FP64, Lieb-chain and sweep rows of the line")
+    ap.add_argument("--precision", type=int, default=32, choices=[32, 64],
+                    help="square64 / lieb384: state precision (64: the reference's, on the LDS kernel)")
     ap.add_argument("--kernel", default="auto", choices=["auto", "stencil64", "fused64", "generic", "mfma", "band"],
                     help="step kernel (auto: the fastest one that supports the workload)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -388,8 +388,8 @@ def main():
     spec = dict(base, t_end=(W + K) * fsteps * base["dt"])
     # the Lieb configuration names the dense GEMM on MFMA: that is its line;
     # the banded kernel (the library's AUTO choice there) is timed beside it
-    kernel = "mfma" if (args.workload == "lieb384" and args.kernel == "auto") else args.kernel
-    eng, info, st, wall = time_frames(Engine(System(**spec), precision=32, kernel=kernel), n_traj, W, K)
+    kernel = "mfma" if (args.workload == "lieb384" and args.kernel == "auto" and args.precision == 32) else args.kernel
+    eng, info, st, wall = time_frames(Engine(System(**spec), precision=args.precision, kernel=kernel), n_traj, W, K)
     res = eng.read()
     if not np.all(np.isfinite(res.sum_n)):
         sys.exit("non-finite ensemble sums")
@@ -415,7 +415,7 @@ def main():
         # path: 2 x 5 N log2 N / N + 30); the nearest-neighbour kernel evaluates
         # the same linear map with fewer executed flops
         flops_site = FLOPS_PER_SITE_STEP
-        peak, bound = PEAK_FP32, "valu"
+        peak, bound = (PEAK_FP64 if args.precision == 64 else PEAK_FP32), "valu"
         kname = info["kernel"] + "_step_kernel"
         basis = ("algorithmic: 150 flops per site-step of the reference's FFT hopping "
                  "(2 x 5 N log2 N / N + 30, N = 4096)")
@@ -426,8 +426,9 @@ def main():
         f"{st['reduce_kernel_ms'] / max(st['reduce_launches'], 1):.3f} ms, grid {st['grid']}")
 
     traffic, counters = None, {}
-    pmc = os.path.join(ROOT, "profiles", f"{info['kernel']}_64x64_16384traj_pmc.json" if args.workload == "square64"
-                       else f"{info['kernel']}_lieb384_8192traj_pmc.json")
+    tag = info["kernel"] + ("_fp64" if args.precision == 64 else "")
+    pmc = os.path.join(ROOT, "profiles", f"{tag}_64x64_16384traj_pmc.json" if args.workload == "square64"
+                       else f"{tag}_lieb384_8192traj_pmc.json")
     if os.path.exists(pmc):
         try:
             prof = json.load(open(pmc))
@@ -444,7 +445,8 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "trajectory·site-steps/s", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == 64 else "fp32",
+        "data": "synthetic",
         "config": {
             "workload": WORKLOAD_TEXT[args.workload],
             "lattice": "64x64" if args.workload == "square64" else "Lieb 128x3 OBC",

@@ -405,6 +405,92 @@ int advance_stack(Ctx* c, const StepArgs& a, hipStream_t s) {
   return TWMC_OK;
 }
 
+// Launch geometry of the step kernel (and of the 64x64 capture kernel) for a
+// batch of B trajectories; allocates nothing.
+struct Geometry {
+  int grid = 0, block = 256, cap_grid = 0;
+  size_t lds = 0;
+};
+
+int step_geometry(Ctx* c, size_t B, Geometry& geo) {
+  int cus = 0;
+  HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  int per_cu = 1;
+  size_t units = B;  // work items of the persistent grid
+  if (c->kernel == TWMC_KERNEL_FUSED64) {
+    geo.block = 64;
+    geo.lds = 0;
+    HIPCHK(c, fused64_occupancy(&per_cu));
+  } else if (c->kernel == TWMC_KERNEL_STENCIL64) {
+    geo.block = 64;
+    geo.lds = 0;
+    int cap_per_cu = 0;
+    HIPCHK(c, capture64_occupancy(&cap_per_cu));
+    if (cap_per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "capture kernel does not fit on a CU");
+    geo.cap_grid = (int)std::min<size_t>(B, (size_t)cap_per_cu * cus);
+    HIPCHK(c, stencil64_occupancy(&per_cu));
+  } else if (c->kernel == TWMC_KERNEL_BAND) {
+    geo.block = 64;
+    geo.lds = 0;
+    HIPCHK(c, band_occupancy(c->n, c->band_w, c->band_lieb, &per_cu));
+  } else if (c->kernel == TWMC_KERNEL_MFMA) {
+    geo.block = 256;
+    geo.lds = mfma_lds_bytes(c->np);
+    units = (B + 31) / 32;
+    HIPCHK(c, mfma_occupancy(c->np, &per_cu));
+  } else {
+    geo.block = generic_block(c->n);
+    geo.lds = generic_lds_bytes(c->p.nx, c->p.ny, c->precision);
+    HIPCHK(c, generic_occupancy(c->precision, c->n, geo.block, geo.lds, &per_cu));
+  }
+  if (per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");
+  if (c->kernel == TWMC_KERNEL_STENCIL64 || c->kernel == TWMC_KERNEL_FUSED64 || c->kernel == TWMC_KERNEL_BAND)
+    geo.grid = balanced_wave_grid(B, per_cu, cus);
+  else
+    geo.grid = (int)std::min<size_t>(units, (size_t)per_cu * cus);
+  return TWMC_OK;
+}
+
+// The step arguments every segment shares: lattice, parameters, plans and
+// the context's device tables; the batch's buffers are set by the caller.
+StepArgs shared_step_args(const Ctx* c) {
+  StepArgs a{};
+  a.nx = c->p.nx;
+  a.ny = c->p.ny;
+  a.n = c->n;
+  a.hopping = c->p.hopping;
+  a.param_stride = c->noisy_params ? c->n : 0;
+  a.dt = c->p.dt;
+  a.dt4 = c->dt4;
+  a.inv_n = 1.0 / double(c->n);
+  a.kscale = -2.0 * c->p.J;
+  a.key0 = uint32_t(c->p.seed & 0xffffffffu);
+  a.key1 = uint32_t(c->p.seed >> 32);
+  a.px = c->px;
+  a.py = c->py;
+  a.twx = c->d_twx.p;
+  a.twy = c->d_twy.p;
+  a.kcx = c->d_kcx.p;
+  a.kcy = c->d_kcy.p;
+  a.csc_ptr = static_cast<const int*>(c->d_cscp.p);
+  a.csc_row = static_cast<const int*>(c->d_cscr.p);
+  a.csc_val = c->d_cscv.p;
+  a.a_hi = c->d_ahi.p;
+  a.a_lo = c->d_alo.p;
+  a.np = c->np;
+  a.ft_n = c->ft_n;
+  a.ft_times = static_cast<const double*>(c->d_ft_times.p);
+  a.ft_vals = c->d_ft_vals.p;
+  a.ft_slopes = c->d_ft_slopes.p;
+  a.u_real = 1;
+  for (int i = 0; i < c->n && a.u_real; ++i) a.u_real = c->base[1][2 * i + 1] == 0.0;
+  if (c->noise[1] != TWMC_NOISE_NONE) a.u_real = 0;
+  a.band_coef = static_cast<const float*>(c->d_band.p);
+  a.band_w = c->band_w;
+  a.band_lieb = c->band_lieb;
+  return a;
+}
+
 int release_batch(Ctx* c) {
   c->batch_active = false;
   c->segs.clear();
@@ -705,39 +791,15 @@ int twmc_batch_begin(twmc_ctx* ctx, const twmc_batch_desc* d) {
   // launch geometry
   int cus = 0;
   HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-  int per_cu = 1;
-  size_t units = B;  // work items of the persistent grid
-  if (c->kernel == TWMC_KERNEL_FUSED64) {
-    c->block = 64;
-    c->lds = 0;
-    HIPCHK(c, fused64_occupancy(&per_cu));
-  } else if (c->kernel == TWMC_KERNEL_STENCIL64) {
-    c->block = 64;
-    c->lds = 0;
-    int cap_per_cu = 0;
-    HIPCHK(c, capture64_occupancy(&cap_per_cu));
-    if (cap_per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "capture kernel does not fit on a CU");
-    c->cap_grid = (int)std::min<size_t>(B, (size_t)cap_per_cu * cus);
-    HIPCHK(c, stencil64_occupancy(&per_cu));
-  } else if (c->kernel == TWMC_KERNEL_BAND) {
-    c->block = 64;
-    c->lds = 0;
-    HIPCHK(c, band_occupancy(c->n, c->band_w, c->band_lieb, &per_cu));
-  } else if (c->kernel == TWMC_KERNEL_MFMA) {
-    c->block = 256;
-    c->lds = mfma_lds_bytes(c->np);
-    units = (B + 31) / 32;
-    HIPCHK(c, mfma_occupancy(c->np, &per_cu));
-  } else {
-    c->block = 256;
-    c->lds = generic_lds_bytes(c->p.nx, c->p.ny, c->precision);
-    HIPCHK(c, generic_occupancy(c->precision, n, c->block, c->lds, &per_cu));
+  {
+    Geometry geo;
+    int rc = step_geometry(c, B, geo);
+    if (rc != TWMC_OK) return rc;
+    c->grid = geo.grid;
+    c->block = geo.block;
+    c->lds = geo.lds;
+    c->cap_grid = geo.cap_grid;
   }
-  if (per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");
-  if (c->kernel == TWMC_KERNEL_STENCIL64 || c->kernel == TWMC_KERNEL_FUSED64 || c->kernel == TWMC_KERNEL_BAND)
-    c->grid = balanced_wave_grid(B, per_cu, cus);
-  else
-    c->grid = (int)std::min<size_t>(units, (size_t)per_cu * cus);
   if (c->kernel == TWMC_KERNEL_STACK) {
     // steps on the sweep kernel; frames stay with the LDS kernel (geometry above).
     // The site-uniform constants, formed as the init kernel forms them
@@ -853,57 +915,24 @@ int twmc_batch_advance(twmc_ctx* ctx, uint64_t n_segments) {
     bool first = true;
     do {
       int steps = (int)std::min<uint64_t>(left, 1u << 30);
-      StepArgs a{};
-      a.nx = c->p.nx;
-      a.ny = c->p.ny;
-      a.n = n;
-      a.hopping = c->p.hopping;
+      StepArgs a = shared_step_args(c);
       a.n_traj = (int)c->n_traj;
-      a.param_stride = c->noisy_params ? n : 0;
-      a.dt = c->p.dt;
-      a.dt4 = c->dt4;
-      a.inv_n = 1.0 / double(n);
-      a.kscale = -2.0 * c->p.J;
-      a.key0 = uint32_t(c->p.seed & 0xffffffffu);
-      a.key1 = uint32_t(c->p.seed >> 32);
       a.step0 = c->desc.step0 + c->steps_done;
       a.n_steps = steps;
       a.capture = (first && sg.capture) ? 1 : 0;
       a.frame = sg.frame < 0 ? 0 : sg.frame;
       a.n_frames = (int)std::max<uint64_t>(c->n_frames, 1);
       a.dump = (c->desc.outputs & TWMC_OUT_TRAJ) ? 1 : 0;
-      a.px = c->px;
-      a.py = c->py;
       a.traj_ids = static_cast<const uint64_t*>(c->d_ids.p);
       a.beta = c->d_beta.p;
       a.rsl = c->d_rsl.p;
       a.U = c->d_U.p;
       a.F = c->d_F.p;
-      a.twx = c->d_twx.p;
-      a.twy = c->d_twy.p;
-      a.kcx = c->d_kcx.p;
-      a.kcy = c->d_kcy.p;
-      a.csc_ptr = static_cast<const int*>(c->d_cscp.p);
-      a.csc_row = static_cast<const int*>(c->d_cscr.p);
-      a.csc_val = c->d_cscv.p;
       a.partials = static_cast<double*>(c->d_partials.p);
       a.traj_m2 = static_cast<double*>(c->d_m2.p);
       a.traj_m4 = static_cast<double*>(c->d_m4.p);
       a.traj_out = c->d_traj.p;
-      a.a_hi = c->d_ahi.p;
-      a.a_lo = c->d_alo.p;
-      a.np = c->np;
-      a.ft_n = c->ft_n;
       a.t0 = c->desc.t_start + double(c->steps_done) * c->p.dt;
-      a.ft_times = static_cast<const double*>(c->d_ft_times.p);
-      a.ft_vals = c->d_ft_vals.p;
-      a.ft_slopes = c->d_ft_slopes.p;
-      a.u_real = 1;
-      for (int i = 0; i < n && a.u_real; ++i) a.u_real = c->base[1][2 * i + 1] == 0.0;
-      if (c->noise[1] != TWMC_NOISE_NONE) a.u_real = 0;
-      a.band_coef = static_cast<const float*>(c->d_band.p);
-      a.band_w = c->band_w;
-      a.band_lieb = c->band_lieb;
       if (c->kernel == TWMC_KERNEL_STENCIL64) {
         int rc = advance_stencil(c, a, s);
         if (rc != TWMC_OK) return rc;

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -185,10 +186,8 @@ __device__ __forceinline__ void step_item(const SweepArgs& g, const SweepPointDe
 }
 
 // Built for three waves per SIMD (three 256-thread workgroups per CU, at most
-// 168 VGPRs): the step loop then spills a few values to scratch, which costs
-// less than the latency the third workgroup hides (a 512-point sweep frame
-// 42.5 -> 34.8 ms against two workgroups per CU at 204 VGPRs; four per CU
-// spill more and lose on stacked 2D items).
+// 168 VGPRs; a 512-point sweep frame 42.5 -> 34.8 ms against two workgroups
+// per CU at 204 VGPRs; four per CU spill and lose on stacked 2D items).
 template <bool CAPTURE_ONLY>
 __global__ void __launch_bounds__(kSweepBlock, 3) sweep_step_kernel(SweepArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -234,33 +233,44 @@ __global__ void __launch_bounds__(kSweepBlock, 3) sweep_step_kernel(SweepArgs g)
 
     // the item's state: this thread's pairs p = t + 256 j in registers
     cpx<float>* gbeta = g.beta + P.state_off + (size_t)item.t0 * nxt * ny;
-    cpx<float> beta[2 * kPPT];
-#pragma unroll
-    for (int j = 0; j < kPPT; ++j) {
-      const int p = threadIdx.x + kSweepBlock * j;
-      if (p < npairs) {
-        beta[2 * j] = gbeta[2 * p];
-        beta[2 * j + 1] = gbeta[2 * p + 1];
-      }
-    }
-
-    if constexpr (!CAPTURE_ONLY) {
-      // the step launch: no capture, neither bufB nor the FFT tables
-      if (!P.fft_x)
-        step_item<false, true>(g, P, item, bufA, beta, 1, ny, npairs);
-      else if (ntr == 1)
-        step_item<true, false>(g, P, item, bufA, beta, nxt, ny, npairs);
-      else
-        step_item<true, true>(g, P, item, bufA, beta, nxt, ny, npairs);
+    auto load_state = [&](cpx<float> (&beta)[2 * kPPT]) __attribute__((always_inline)) {
 #pragma unroll
       for (int j = 0; j < kPPT; ++j) {
         const int p = threadIdx.x + kSweepBlock * j;
         if (p < npairs) {
-          gbeta[2 * p] = beta[2 * j];
-          gbeta[2 * p + 1] = beta[2 * j + 1];
+          beta[2 * j] = gbeta[2 * p];
+          beta[2 * j + 1] = gbeta[2 * p + 1];
         }
       }
+    };
+
+    if constexpr (!CAPTURE_ONLY) {
+      // the step launch: no capture, neither bufB nor the FFT tables.  Each
+      // kind of item loads, steps and stores its own copy of the state, so no
+      // register array is live across the three bodies (one shared copy made
+      // the register allocator spill ~26 values per lane at this occupancy)
+      auto run = [&](auto xhop, auto multi, int rows) __attribute__((always_inline)) {
+        cpx<float> beta[2 * kPPT];
+        load_state(beta);
+        step_item<decltype(xhop)::value, decltype(multi)::value>(g, P, item, bufA, beta, rows, ny, npairs);
+#pragma unroll
+        for (int j = 0; j < kPPT; ++j) {
+          const int p = threadIdx.x + kSweepBlock * j;
+          if (p < npairs) {
+            gbeta[2 * p] = beta[2 * j];
+            gbeta[2 * p + 1] = beta[2 * j + 1];
+          }
+        }
+      };
+      if (!P.fft_x)
+        run(std::false_type{}, std::true_type{}, 1);
+      else if (ntr == 1)
+        run(std::true_type{}, std::false_type{}, nxt);
+      else
+        run(std::true_type{}, std::true_type{}, nxt);
     } else {
+      cpx<float> beta[2 * kPPT];
+      load_state(beta);
       // the capture launch records the frame: per-trajectory means, mean
       // field and k-space bins of the state
 #pragma unroll

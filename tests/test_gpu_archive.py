@@ -1,26 +1,10 @@
 """The archive twmc_sim writes, checked byte for byte against the reference's
-write sequence, and the worker pool on the GPU.
-
-Expected bytes are assembled here from the reference's writer, independently
-of the project's own reader (pincosimulator_amd/pinco_format.py):
-  * _register.bin header: ChunkRegister::InitializeRegisterHeader
-    (src/Base/FileFormats/PincoFormat/ChunkRegister.cpp:152-233) -- 8 magic
-    bytes, the version byte, fputc(dataset count), per dataset fputc(name
-    length) + name, per dataset size_t D + D size_t dims, per dataset
-    fputc(format), long extra-data size, long offset of the first entry (the
-    position right after that field);
-  * entries: ChunkRegister::RegisterStoredData (:312-381) -- 4 size_t {id,
-    chunk id, index in the chunk, continuation offset} + the extra data
-    {t_start, t_final} (TWMCBaseSolver.cpp:278-279); a continued trajectory's
-    previous entry gets the register position of its new entry;
-  * variable<i>_<chunk>.cnk: ChunkFileSet::WriteDataToChunk
-    (ChunkFileSet.cpp:147-160) -- size_t byte count + raw bytes per entry;
-  * index_<chunk>.bin: ChunkFileSet::WriteToChunk (:111-145, record size
-    :93-94) -- size_t id + {elements, offset} per dataset;
-  * a new run folder's first chunk is 0 and a later run's is the largest used
-    id + 1 (PincoFormatDataStore.cpp:40-57; the reference's max over an empty
-    set is undefined, this writer starts at 0).
-The payload is checked against the FP64 oracle to the FP64 bar (1e-9)."""
+write sequence (tests/pinco_bytes.py, assembled independently of the
+project's own reader), and the worker pool on the GPU.  A new run folder's
+first chunk is 0 and a later run's is the largest used id + 1
+(PincoFormatDataStore.cpp:40-57; the reference's max over an empty set is
+undefined, this writer starts at 0).  The payload is checked against the FP64
+oracle to the FP64 bar (1e-9)."""
 import os
 import struct
 
@@ -29,28 +13,12 @@ import pytest
 
 import oracle
 from helpers import rel_err
+from pinco_bytes import ENTRY_BYTES, entry, header
 from test_gpu_cli import CHAIN, run_sim
 
 pytestmark = pytest.mark.gpu
 
-MAGIC = bytes([0x89, 0x50, 0x4E, 0x43, 0x0D, 0x0A, 0x1A, 0x0A])
 N, DT, FS = 16, 0.005, 40
-
-
-def header(names, dims, formats):
-    b = bytearray(MAGIC) + bytes([1, len(names)])
-    for n in names:
-        b += bytes([len(n)]) + n.encode()
-    for d in dims:
-        b += struct.pack("<Q", len(d)) + struct.pack("<%dQ" % len(d), *d)
-    b += bytes(formats)
-    b += struct.pack("<q", 16)
-    b += struct.pack("<q", len(b) + 8)
-    return bytes(b)
-
-
-def entry(tid, chunk, index, cont, t0, t1):
-    return struct.pack("<4Q2d", tid, chunk, index, cont, t0, t1)
 
 
 def spec(t_end):
@@ -101,7 +69,7 @@ def test_register_chunks_and_index_bytes_with_a_continuation(tmp_path):
                                           init_mode=oracle.INIT_PREVIOUS, beta_prev=prev_final,
                                           store_initial=False, step0=step0)
         f2[tid], tf2[tid] = fr, tf
-    esz = 48
+    esz = ENTRY_BYTES
     reg = open(os.path.join(folder, "_register.bin"), "rb").read()
     assert reg == (hdr + entry(1, 0, 0, begin + 2 * esz, 0.0, tf1[1]) + entry(2, 0, 1, begin + 3 * esz, 0.0, tf1[2])
                    + entry(1, 1, 0, 0, tf1[1], tf2[1]) + entry(2, 1, 1, 0, tf1[2], tf2[2]))
