@@ -439,7 +439,7 @@ int step_geometry(Ctx* c, size_t B, Geometry& geo) {
     units = (B + 31) / 32;
     HIPCHK(c, mfma_occupancy(c->np, &per_cu));
   } else {
-    geo.block = generic_block(c->n);
+    geo.block = 256;
     geo.lds = generic_lds_bytes(c->p.nx, c->p.ny, c->precision);
     HIPCHK(c, generic_occupancy(c->precision, c->n, geo.block, geo.lds, &per_cu));
   }

@@ -20,6 +20,8 @@ namespace twmc {
 
 namespace {
 
+constexpr int kBlock = 256;
+
 template <int S, typename T>
 __device__ cpx<T>* fft2d(cpx<T>* a, cpx<T>* b, const StepArgs& g, const cpx<T>* twx, const cpx<T>* twy) {
   // rows: length ny, contiguous; then columns: length nx, stride ny
@@ -46,17 +48,12 @@ __device__ __forceinline__ double warp_sum(double v) {
   return v;
 }
 
-// BLOCK threads own the site pairs p = t + BLOCK j, j < PPT: 256 threads, or
-// 512 for lattices of 2049..4096 sites (four pairs per thread instead of
-// eight, two waves per SIMD instead of one: the FP64 kernel keeps ~390
-// registers per wave at eight pairs)
-template <typename T, int PPT, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
-  constexpr int kBlock = BLOCK;
+template <typename T, int PPT>
+__global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = g.n;
-  double* red = reinterpret_cast<double*>(smem);  // 2 doubles per wave: per-wave partial sums
-  cpx<T>* bufA = reinterpret_cast<cpx<T>*>(smem + kGenericRedBytes);
+  double* red = reinterpret_cast<double*>(smem);  // 8 doubles: per-wave partial sums
+  cpx<T>* bufA = reinterpret_cast<cpx<T>*>(smem + 64);
   cpx<T>* bufB = bufA + n;
   cpx<T>* twx = bufB + n;
   cpx<T>* twy = twx + g.nx;
@@ -457,34 +454,30 @@ __global__ void reduce_kernel(ReduceArgs a) {
   }
 }
 
-template <typename T, int PPT, int BLOCK>
-hipError_t launch_generic_t(const StepArgs& a, int grid, size_t lds, hipStream_t s) {
-  if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void*)generic_step_kernel<T, PPT, BLOCK>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL((generic_step_kernel<T, PPT, BLOCK>), dim3(grid), dim3(BLOCK), lds, s, a);
+template <typename T, int PPT>
+hipError_t launch_generic_t(const StepArgs& a, int grid, int block, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL((generic_step_kernel<T, PPT>), dim3(grid), dim3(block), lds, s, a);
   return hipGetLastError();
 }
 
 template <typename T>
-const void* generic_kernel_ptr(int ppt, int block) {
-  if (block == 512) return (const void*)generic_step_kernel<T, 4, 512>;
-  return ppt == 1   ? (const void*)generic_step_kernel<T, 1, 256>
-         : ppt == 2 ? (const void*)generic_step_kernel<T, 2, 256>
-         : ppt == 8 ? (const void*)generic_step_kernel<T, 8, 256>
-                    : (const void*)generic_step_kernel<T, 32, 256>;
-}
-
-template <typename T>
 hipError_t launch_generic_p(const StepArgs& a, int grid, int block, size_t lds, hipStream_t s) {
-  if (block == 512) return launch_generic_t<T, 4, 512>(a, grid, lds, s);
-  switch (generic_pairs_per_thread(a.n, block)) {
-    case 1: return launch_generic_t<T, 1, 256>(a, grid, lds, s);
-    case 2: return launch_generic_t<T, 2, 256>(a, grid, lds, s);
-    case 8: return launch_generic_t<T, 8, 256>(a, grid, lds, s);
-    default: return launch_generic_t<T, 32, 256>(a, grid, lds, s);
+  int ppt = generic_pairs_per_thread(a.n, block);
+  if (lds > 65536) {
+    hipError_t e;
+    switch (ppt) {
+      case 1: e = hipFuncSetAttribute((const void*)generic_step_kernel<T, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); break;
+      case 2: e = hipFuncSetAttribute((const void*)generic_step_kernel<T, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); break;
+      case 8: e = hipFuncSetAttribute((const void*)generic_step_kernel<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); break;
+      default: e = hipFuncSetAttribute((const void*)generic_step_kernel<T, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); break;
+    }
+    if (e != hipSuccess) return e;
+  }
+  switch (ppt) {
+    case 1: return launch_generic_t<T, 1>(a, grid, block, lds, s);
+    case 2: return launch_generic_t<T, 2>(a, grid, block, lds, s);
+    case 8: return launch_generic_t<T, 8>(a, grid, block, lds, s);
+    default: return launch_generic_t<T, 32>(a, grid, block, lds, s);
   }
 }
 
@@ -493,7 +486,6 @@ hipError_t launch_generic_p(const StepArgs& a, int grid, int block, size_t lds, 
 int generic_pairs_per_thread(int n, int block) {
   int np = (n + 1) / 2;
   int need = (np + block - 1) / block;
-  if (block == 512) return need <= 4 ? 4 : -1;
   if (need <= 1) return 1;
   if (need <= 2) return 2;
   if (need <= 8) return 8;
@@ -501,21 +493,27 @@ int generic_pairs_per_thread(int n, int block) {
   return -1;
 }
 
-int generic_block(int n) {
-  const int np = (n + 1) / 2;
-  return (np > 1024 && np <= 2048) ? 512 : 256;
-}
-
 size_t generic_lds_bytes(int nx, int ny, int precision) {
   size_t es = precision == 64 ? 16 : 8;
   size_t rs = precision == 64 ? 8 : 4;
   size_t n = (size_t)nx * ny;
-  return kGenericRedBytes + 2 * n * es + (size_t)(nx + ny) * es + (size_t)(nx + ny) * rs;
+  return 64 + 2 * n * es + (size_t)(nx + ny) * es + (size_t)(nx + ny) * rs;
 }
 
 hipError_t generic_occupancy(int precision, int n, int block, size_t lds, int* blocks_per_cu) {
-  const int ppt = generic_pairs_per_thread(n, block);
-  const void* f = precision == 64 ? generic_kernel_ptr<double>(ppt, block) : generic_kernel_ptr<float>(ppt, block);
+  int ppt = generic_pairs_per_thread(n, block);
+  const void* f;
+  if (precision == 64) {
+    f = ppt == 1 ? (const void*)generic_step_kernel<double, 1>
+        : ppt == 2 ? (const void*)generic_step_kernel<double, 2>
+        : ppt == 8 ? (const void*)generic_step_kernel<double, 8>
+                   : (const void*)generic_step_kernel<double, 32>;
+  } else {
+    f = ppt == 1 ? (const void*)generic_step_kernel<float, 1>
+        : ppt == 2 ? (const void*)generic_step_kernel<float, 2>
+        : ppt == 8 ? (const void*)generic_step_kernel<float, 8>
+                   : (const void*)generic_step_kernel<float, 32>;
+  }
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;

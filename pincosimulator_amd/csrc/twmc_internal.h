@@ -172,11 +172,8 @@ hipError_t launch_gauss_init(const GaussArgs& a, int precision, int grid, hipStr
 size_t gauss_lds_bytes(int nx, int ny, int precision);
 hipError_t launch_generic(const StepArgs& a, int precision, int grid, int block, size_t lds,
                           hipStream_t s);
-// bytes at the start of the LDS kernel's shared memory for per-wave sums (2 doubles x up to 8 waves)
-constexpr int kGenericRedBytes = 128;
 size_t generic_lds_bytes(int nx, int ny, int precision);
 int generic_pairs_per_thread(int n, int block);
-int generic_block(int n);  // threads per workgroup of the LDS kernel for n sites
 hipError_t generic_occupancy(int precision, int n, int block, size_t lds, int* blocks_per_cu);
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
 
