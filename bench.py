@@ -353,7 +353,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="sweep512: keep the default stacking instead of autotuning it first")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run one 256 MiB device-to-device copy (torch) whose HBM counters "
+                    help="first run one 256 MiB device-to-device copy (hipMemcpy) whose HBM counters "
                          "calibrate FETCH_SIZE / WRITE_SIZE in a --pmc profile")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -368,14 +368,16 @@ def main():
 
     W, K = args.warmup, args.steps
     if args.copy_calibration:
-        import torch
-        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
+        # through the HIP runtime libtwmc_hip.so uses (no second runtime in the process)
+        from pincosimulator_amd.device import DeviceBuffer, synchronize
+        a = DeviceBuffer.from_array(np.ones(64 << 20, dtype=np.float32))
+        b = DeviceBuffer(a.nbytes)
+        synchronize()
+        b.copy_from(a)
+        synchronize()
         log("bench: calibration copy of 256 MiB done")
-        del a, b
+        a.free()
+        b.free()
     if args.workload == "sweep512":
         return bench_sweep(args)
     if args.workload == "warmstart64":

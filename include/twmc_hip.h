@@ -192,6 +192,32 @@ int twmc_reset_stats(twmc_ctx* ctx);
  * the owning thread advances or reads the batch */
 float twmc_progress(const twmc_ctx* ctx);
 
+/* ---- allocation-free stepping --------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory while they run (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  A variant of twmc_batch_advance on
+ * caller-owned device buffers that calls no allocating routine:
+ *   d_traj_ids  n_traj trajectory ids (device)
+ *   d_beta      n_traj x N complex state of the context's precision (device,
+ *               interleaved, float or double), advanced in place by n_steps
+ *               Euler steps whose first has absolute index step0 (the Philox
+ *               counter: continuing a trajectory means step0 += n_steps)
+ *   d_sums      5 x N doubles (device) or NULL: per-site sums over the batch of
+ *               the state after the steps, rows sum |b|^2, sum |b|^4, 0 (no
+ *               k-space sums), sum Re b, sum Im b
+ *   stream      hipStream_t to enqueue on, NULL: the context's stream
+ * Asynchronous; no host synchronisation.  Site-shared parameters and a static
+ * drive only, on the context's kernel (the stacked kernel excepted: create
+ * such a context with TWMC_KERNEL_GENERIC).  Its own device memory -- the
+ * parameter tables and the partial sums -- is one fixed array of
+ * TWMC_NOALLOC_SCRATCH_BYTES held by every GPU the library runs on until the
+ * process ends; a lattice whose tables and one partial slot exceed it is
+ * rejected.  Every call on one GPU shares that array, so calls that overlap
+ * on one GPU (different streams, threads or contexts) race. */
+#define TWMC_NOALLOC_SCRATCH_BYTES (32u << 20)
+int twmc_advance_noalloc(twmc_ctx* ctx, const uint64_t* d_traj_ids, uint64_t n_traj, uint64_t step0,
+                         uint64_t n_steps, void* d_beta, double* d_sums, void* stream);
+
 /* Host-side NoisyMatrix realisation with the kernels' Philox layout (tag as
  * in twmc_rng.h).  Used for one shared disorder realisation. */
 int twmc_noisy_matrix(const twmc_matrix* m, uint64_t n, uint64_t seed, uint64_t traj,

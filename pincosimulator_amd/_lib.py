@@ -28,6 +28,7 @@ EXPORTS = [
     "twmc_abi_version", "twmc_last_error", "twmc_create", "twmc_destroy", "twmc_batch_begin",
     "twmc_batch_info_get", "twmc_batch_advance", "twmc_batch_sync", "twmc_batch_read", "twmc_run",
     "twmc_set_timing", "twmc_query", "twmc_reset_stats", "twmc_progress", "twmc_noisy_matrix",
+    "twmc_advance_noalloc",
     "twmc_sweep_create", "twmc_sweep_destroy", "twmc_sweep_last_error", "twmc_sweep_set_batching",
     "twmc_sweep_get_batching", "twmc_sweep_autotune", "twmc_sweep_run", "twmc_sweep_query",
     "twmc_sweep_mode_bins",
@@ -172,9 +173,10 @@ def load():
     for name in ("twmc_mlp_create", "twmc_mlp_get_params", "twmc_mlp_set_params", "twmc_mlp_forward",
                  "twmc_mlp_loss_grad", "twmc_mlp_train"):
         getattr(L, name).restype = C.c_int
+    L.twmc_advance_noalloc.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, vp, vp]
     for name in ("twmc_create", "twmc_batch_begin", "twmc_batch_info_get", "twmc_batch_advance",
                  "twmc_batch_sync", "twmc_batch_read", "twmc_run", "twmc_set_timing", "twmc_query",
-                 "twmc_reset_stats", "twmc_noisy_matrix"):
+                 "twmc_reset_stats", "twmc_noisy_matrix", "twmc_advance_noalloc"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

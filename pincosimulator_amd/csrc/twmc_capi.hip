@@ -23,7 +23,17 @@
 
 using namespace twmc;
 
+// Device memory of twmc_advance_noalloc beyond the caller's buffers (the
+// site-shared parameter tables and the per-chunk partial sums): one
+// fixed-size array, of which every GPU the library runs on holds its own copy
+// until the process ends.  Host code uses it only through the per-device
+// address from hipGetSymbolAddress (g_scratch_addr), never through its name.
+static __device__ __align__(256) unsigned char scratch[TWMC_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_addr;  // device address of `scratch` per device index
 
 thread_local std::string g_last_error;
 
@@ -405,6 +415,24 @@ int advance_stack(Ctx* c, const StepArgs& a, hipStream_t s) {
   return TWMC_OK;
 }
 
+// The address of this device's copy of `scratch`, looked up once per device
+// (the device must be current).
+hipError_t scratch_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((size_t)device < g_scratch_addr.size() && g_scratch_addr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (g_scratch_addr.size() <= (size_t)device) g_scratch_addr.resize(device + 1, nullptr);
+  g_scratch_addr[device] = ptr;
+  return hipSuccess;
+}
+
+void* scratch_address(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (size_t)device < g_scratch_addr.size() ? g_scratch_addr[device] : nullptr;
+}
+
 // Launch geometry of the step kernel (and of the 64x64 capture kernel) for a
 // batch of B trajectories; allocates nothing.
 struct Geometry {
@@ -589,6 +617,13 @@ int twmc_create(const twmc_params* p, int device, twmc_ctx** ctx_out) {
   if (e != hipSuccess) {
     delete c;
     return hip_fail(nullptr, e, "hipStreamCreate");
+  }
+  // this device's copy of the allocation-free entry point's scratch array
+  e = scratch_setup(device);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return hip_fail(nullptr, e, "hipGetSymbolAddress(scratch)");
   }
 
   // constant tables, computed as the reference computes them
@@ -1120,6 +1155,114 @@ float twmc_progress(const twmc_ctx* ctx) {
     c->seg_completed++;
   if (c->seg_completed == 0 || c->n_steps == 0) return 0.f;
   return float(double(c->seg_steps[c->seg_completed - 1]) / double(c->n_steps));
+}
+
+int twmc_advance_noalloc(twmc_ctx* ctx, const uint64_t* d_traj_ids, uint64_t n_traj, uint64_t step0,
+                         uint64_t n_steps, void* d_beta, double* d_sums, void* stream) {
+  if (!ctx || !d_traj_ids || !d_beta) return fail(nullptr, TWMC_ERR_INVALID, "null argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (n_traj == 0) return fail(c, TWMC_ERR_INVALID, "empty trajectory batch");
+  if (n_traj > (uint64_t)INT32_MAX) return fail(c, TWMC_ERR_INVALID, "batch too large");
+  if (c->noisy_params)
+    return fail(c, TWMC_ERR_UNSUPPORTED, "twmc_advance_noalloc needs site-shared parameters (no per-trajectory disorder)");
+  if (c->ft_n > 0) return fail(c, TWMC_ERR_UNSUPPORTED, "twmc_advance_noalloc needs a static drive");
+  if (c->kernel == TWMC_KERNEL_STACK)
+    return fail(c, TWMC_ERR_UNSUPPORTED,
+                "twmc_advance_noalloc does not run the stacked kernel: create the context with kernel = GENERIC");
+  HIPCHK(c, hipSetDevice(c->device));
+  unsigned char* base = static_cast<unsigned char*>(scratch_address(c->device));
+  if (!base) return fail(c, TWMC_ERR_STATE, "no scratch address for this device");
+  const int n = c->n;
+  // scratch layout: rsl, U, F tables (n complex each), then [chunks][kQ][n] doubles
+  const size_t table = ((size_t)n * c->es + 255) / 256 * 256;
+  const size_t params = 3 * table;
+  const size_t per_chunk = (size_t)kQ * n * sizeof(double);
+  if (params + per_chunk > (size_t)TWMC_NOALLOC_SCRATCH_BYTES)
+    return fail(c, TWMC_ERR_INVALID, "lattice too large for the allocation-free scratch array");
+  const int max_chunks =
+      (int)std::min<size_t>(kSiteSumChunks, ((size_t)TWMC_NOALLOC_SCRATCH_BYTES - params) / per_chunk);
+  Geometry geo;
+  int rc = step_geometry(c, n_traj, geo);
+  if (rc != TWMC_OK) return rc;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  void* rsl = base;
+  void* U = base + table;
+  void* F = base + 2 * table;
+  double* partials = reinterpret_cast<double*>(base + params);
+
+  // the site-shared parameter tables, as twmc_batch_begin forms them (the
+  // init kernel with one parameter set, writing no state)
+  InitArgs ia{};
+  ia.n = n;
+  ia.n_traj = 1;
+  ia.mode = TWMC_INIT_GAUSSIAN;
+  ia.per_traj_params = 0;
+  ia.onsite = c->p.onsite;
+  ia.gamma = c->p.gamma;
+  ia.detuning = c->p.detuning;
+  ia.key0 = uint32_t(c->p.seed & 0xffffffffu);
+  ia.key1 = uint32_t(c->p.seed >> 32);
+  ia.traj_ids = d_traj_ids;
+  for (int k = 0; k < 5; ++k) {
+    ia.base[k] = static_cast<const double*>(c->d_base[k].p);
+    ia.amp[k] = static_cast<const double*>(c->d_amp[k].p);
+    ia.noise[k] = c->noise[k];
+  }
+  ia.beta = d_beta;
+  ia.rsl = rsl;
+  ia.U = U;
+  ia.F = F;
+  HIPCHK(c, launch_init(ia, c->precision, s));
+
+  uint64_t done = 0;
+  while (done < n_steps) {
+    const int steps = (int)std::min<uint64_t>(n_steps - done, 1u << 30);
+    StepArgs a = shared_step_args(c);
+    a.n_traj = (int)n_traj;
+    a.step0 = step0 + done;
+    a.n_steps = steps;
+    a.n_frames = 1;
+    a.traj_ids = d_traj_ids;
+    a.beta = d_beta;
+    a.rsl = rsl;
+    a.U = U;
+    a.F = F;
+    a.partials = partials;
+    a.t0 = double(step0 + done) * c->p.dt;
+    if (c->kernel == TWMC_KERNEL_STENCIL64) {
+      HIPCHK(c, launch_stencil64(a, geo.grid, s));
+    } else if (c->kernel == TWMC_KERNEL_BAND) {
+      HIPCHK(c, launch_band(a, geo.grid, s));
+    } else if (c->kernel == TWMC_KERNEL_FUSED64) {
+      HIPCHK(c, launch_fused64(a, geo.grid, s));
+    } else if (c->kernel == TWMC_KERNEL_MFMA) {
+      HIPCHK(c, launch_mfma(a, geo.grid, s));
+    } else {
+      HIPCHK(c, launch_generic(a, c->precision, geo.grid, geo.block, geo.lds, s));
+    }
+    done += (uint64_t)steps;
+  }
+  if (d_sums) {
+    // per-site sums of the final state: every row of every chunk slot is
+    // written (row 2 with zeros), then folded in slot order, read-only
+    SiteSumArgs ss{};
+    ss.beta = d_beta;
+    ss.partials = partials;
+    ss.n = n;
+    ss.n_traj = (int)n_traj;
+    ss.zero_k = 1;
+    const int chunks = site_sum_chunks(n_traj, max_chunks, &ss.chunk);
+    HIPCHK(c, launch_site_sums(ss, c->precision, s));
+    ReduceArgs r{};
+    r.n = n;
+    r.slots = chunks;
+    r.keep = 1;
+    r.partials_in = partials;
+    r.partials = partials;
+    r.obs = d_sums;
+    HIPCHK(c, launch_reduce(r, s));
+  }
+  return TWMC_OK;
 }
 
 int twmc_noisy_matrix(const twmc_matrix* m, uint64_t n, uint64_t seed, uint64_t traj, uint32_t tag,
