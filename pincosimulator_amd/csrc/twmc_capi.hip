@@ -109,6 +109,8 @@ struct Ctx {
   mutable size_t seg_completed = 0;
   int grid = 0, block = 256;
   int cap_grid = 0;  // stencil64: grid of the capture launch (the 64x64 FFT kernel's)
+  int grid_nn = 0;   // LDS kernel, FFT hopping: grid of the nearest-neighbour step launch
+  size_t lds_nn = 0;
   size_t lds = 0;
   // timing
   bool timing = false;
@@ -438,6 +440,8 @@ void* scratch_address(int device) {
 struct Geometry {
   int grid = 0, block = 256, cap_grid = 0;
   size_t lds = 0;
+  int grid_nn = 0;  // LDS kernel with FFT hopping: the step launch without the FFT's second buffer
+  size_t lds_nn = 0;
 };
 
 int step_geometry(Ctx* c, size_t B, Geometry& geo) {
@@ -462,7 +466,7 @@ int step_geometry(Ctx* c, size_t B, Geometry& geo) {
     geo.lds = 0;
     HIPCHK(c, band_occupancy(c->n, c->band_w, c->band_lieb, &per_cu));
   } else if (c->kernel == TWMC_KERNEL_MFMA) {
-    geo.block = 256;
+    geo.block = mfma_block(c->np);
     geo.lds = mfma_lds_bytes(c->np);
     units = (B + 31) / 32;
     HIPCHK(c, mfma_occupancy(c->np, &per_cu));
@@ -470,6 +474,13 @@ int step_geometry(Ctx* c, size_t B, Geometry& geo) {
     geo.block = 256;
     geo.lds = generic_lds_bytes(c->p.nx, c->p.ny, c->precision);
     HIPCHK(c, generic_occupancy(c->precision, c->n, geo.block, geo.lds, &per_cu));
+    if (c->p.hopping == TWMC_HOP_FFT) {
+      int per_cu_nn = 0;
+      geo.lds_nn = generic_nn_lds_bytes(c->n, c->precision);
+      HIPCHK(c, generic_nn_occupancy(c->precision, c->n, geo.lds_nn, &per_cu_nn));
+      if (per_cu_nn < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");
+      geo.grid_nn = (int)std::min<size_t>(B, (size_t)per_cu_nn * cus);
+    }
   }
   if (per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");
   if (c->kernel == TWMC_KERNEL_STENCIL64 || c->kernel == TWMC_KERNEL_FUSED64 || c->kernel == TWMC_KERNEL_BAND)
@@ -517,6 +528,39 @@ StepArgs shared_step_args(const Ctx* c) {
   a.band_w = c->band_w;
   a.band_lieb = c->band_lieb;
   return a;
+}
+
+// LDS-kernel segment with FFT hopping: the capture (if any) is the kernel's
+// capture-only launch (the FFT for the k-space sums, two LDS copies of the
+// state) and the reduction; the steps are the nearest-neighbour launch, one
+// LDS copy, so more workgroups fit on a CU
+int advance_generic_fft(Ctx* c, const StepArgs& a, hipStream_t s) {
+  const int n = c->n;
+  if (a.capture) {
+    StepArgs ca = a;
+    ca.n_steps = 0;
+    hipEvent_t r0 = nullptr;
+    HIPCHK(c, timed(c, c->ev_red, s, true, r0));
+    HIPCHK(c, launch_generic(ca, c->precision, c->grid, c->block, c->lds, s));
+    ReduceArgs r{};
+    r.n = n;
+    r.slots = c->grid;
+    r.partials_in = static_cast<const double*>(c->d_partials.p);
+    r.partials = static_cast<double*>(c->d_partials.p);
+    r.obs = static_cast<double*>(c->d_obs.p) + (size_t)a.frame * kQ * n;
+    HIPCHK(c, launch_reduce(r, s));
+    HIPCHK(c, timed(c, c->ev_red, s, false, r0));
+  }
+  if (a.n_steps > 0) {
+    StepArgs sa = a;
+    sa.capture = 0;
+    hipEvent_t e0 = nullptr;
+    HIPCHK(c, timed(c, c->ev_step, s, true, e0));
+    HIPCHK(c, launch_generic_nn(sa, c->precision, c->grid_nn, c->lds_nn, s));
+    HIPCHK(c, timed(c, c->ev_step, s, false, e0));
+    c->stats.site_steps += (uint64_t)a.n_steps * (uint64_t)n * c->n_traj;
+  }
+  return TWMC_OK;
 }
 
 int release_batch(Ctx* c) {
@@ -834,6 +878,8 @@ int twmc_batch_begin(twmc_ctx* ctx, const twmc_batch_desc* d) {
     c->block = geo.block;
     c->lds = geo.lds;
     c->cap_grid = geo.cap_grid;
+    c->grid_nn = geo.grid_nn;
+    c->lds_nn = geo.lds_nn;
   }
   if (c->kernel == TWMC_KERNEL_STACK) {
     // steps on the sweep kernel; frames stay with the LDS kernel (geometry above).
@@ -976,6 +1022,9 @@ int twmc_batch_advance(twmc_ctx* ctx, uint64_t n_segments) {
         if (rc != TWMC_OK) return rc;
       } else if (c->kernel == TWMC_KERNEL_STACK) {
         int rc = advance_stack(c, a, s);
+        if (rc != TWMC_OK) return rc;
+      } else if (c->kernel == TWMC_KERNEL_GENERIC && c->p.hopping == TWMC_HOP_FFT) {
+        int rc = advance_generic_fft(c, a, s);
         if (rc != TWMC_OK) return rc;
       } else if (a.n_steps > 0 || a.capture) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1237,6 +1286,8 @@ int twmc_advance_noalloc(twmc_ctx* ctx, const uint64_t* d_traj_ids, uint64_t n_t
       HIPCHK(c, launch_fused64(a, geo.grid, s));
     } else if (c->kernel == TWMC_KERNEL_MFMA) {
       HIPCHK(c, launch_mfma(a, geo.grid, s));
+    } else if (c->p.hopping == TWMC_HOP_FFT) {
+      HIPCHK(c, launch_generic_nn(a, c->precision, geo.grid_nn, geo.lds_nn, s));
     } else {
       HIPCHK(c, launch_generic(a, c->precision, geo.grid, geo.block, geo.lds, s));
     }

@@ -1,8 +1,9 @@
 // twmc_generic.hip -- LDS-resident TWMC step kernel for a lattice of any
 // size (mixed-radix FFT: radices 2, 3, 4, 5, 8 and a direct-sum pass for each
-// larger prime factor) and for matrix hopping, FP32 or FP64.  One workgroup advances one trajectory at a time; a persistent grid
-// walks the batch.  This kernel is the FP64 parity path and the fallback for
-// shapes the register-resident kernel does not cover.
+// larger prime factor) and for matrix hopping, FP32 or FP64.  One workgroup
+// advances one trajectory at a time; a persistent grid walks the batch.  This
+// kernel is the FP64 path and the fallback for shapes the register-resident
+// kernels do not cover.
 //
 // Per step (src/TWMC/Solvers/TWMCBaseSolver.cpp:232-256):
 //   beta -> LDS, forward DFT (rows, then columns; Stockham, ping-pong),
@@ -48,7 +49,10 @@ __device__ __forceinline__ double warp_sum(double v) {
   return v;
 }
 
-template <typename T, int PPT>
+// NN: a step launch with the hopping in its nearest-neighbour form and no
+// frame capture (the capture runs as its own launch): the kernel then needs
+// one LDS copy of the state and none of the FFT's registers.
+template <typename T, int PPT, bool NN = false>
 __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = g.n;
@@ -60,19 +64,21 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
   T* kcx = reinterpret_cast<T*>(twy + g.ny);
   T* kcy = kcx + g.nx;
 
-  for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
-    twx[i] = reinterpret_cast<const cpx<T>*>(g.twx)[i];
-    kcx[i] = reinterpret_cast<const T*>(g.kcx)[i];
+  if constexpr (!NN) {
+    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+      twx[i] = reinterpret_cast<const cpx<T>*>(g.twx)[i];
+      kcx[i] = reinterpret_cast<const T*>(g.kcx)[i];
+    }
+    for (int i = threadIdx.x; i < g.ny; i += blockDim.x) {
+      twy[i] = reinterpret_cast<const cpx<T>*>(g.twy)[i];
+      kcy[i] = reinterpret_cast<const T*>(g.kcy)[i];
+    }
+    __syncthreads();
   }
-  for (int i = threadIdx.x; i < g.ny; i += blockDim.x) {
-    twy[i] = reinterpret_cast<const cpx<T>*>(g.twy)[i];
-    kcy[i] = reinterpret_cast<const T*>(g.kcy)[i];
-  }
-  __syncthreads();
 
   const int npairs = (n + 1) / 2;
   double* part = g.partials + (size_t)blockIdx.x * kQ * n;
-  constexpr bool kStencil = sizeof(T) == 4;
+  constexpr bool kStencil = sizeof(T) == 4 || NN;
   // x = floor(i / ny) as umulhi(i, ceil(2^32 / ny)): exact for i ny < 2^32
   const uint32_t ymagic = g.ny > 1 ? (uint32_t)((0x100000000ull + (uint64_t)g.ny - 1) / (uint64_t)g.ny) : 0u;
   const T jval = T(-0.5 * g.kscale);  // J
@@ -110,9 +116,12 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
       }
       __syncthreads();
       cpx<T>* cur = bufA;
-      const bool cap = (s == 0 && g.capture);
-      // FP64 applies the reference's FFT form every step; FP32 uses its exact
-      // nearest-neighbour form and transforms only for a frame's k-space sums
+      const bool cap = !NN && (s == 0 && g.capture);
+      // FP32 (and the FP64 step launch, NN) uses the FFT form's exact
+      // nearest-neighbour equivalent and transforms only for a frame's k-space
+      // sums; an FP64 launch with steps and FFT hopping without NN applies the
+      // FFT form every step (the host issues steps of FP64 FFT-hopping
+      // batches as NN launches, captures as capture-only launches of this one)
       if (g.hopping == 0 && (!kStencil || cap)) cur = fft2d<-1>(bufA, bufB, g, twx, twy);
 
       if (cap) {
@@ -168,7 +177,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
       }
       if (s >= g.n_steps) break;
 
-      if (g.hopping == 0 && kStencil) {
+      if (NN || (g.hopping == 0 && kStencil)) {
         if (cap) {
           // the transform overwrote bufA: stage beta again for the neighbour sums
 #pragma unroll
@@ -182,7 +191,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
           __syncthreads();
         }
         cur = bufA;
-      } else if (g.hopping == 0) {
+      } else if (!NN && g.hopping == 0) {
         // K product: K = i kscale (fx cos kx + fy cos ky) (TWMCBaseSolver.cpp:80-88, :240)
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
           int kx = i / g.ny, ky = i - kx * g.ny;
@@ -193,7 +202,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
         __syncthreads();
         cpx<T>* other = (cur == bufA) ? bufB : bufA;
         cur = fft2d<+1>(cur, other, g, twx, twy);
-      } else {
+      } else if (!NN) {
         // h = -i (beta . C) over the nonzeros of each column (TWMCLiebSolver.cpp:262)
         const T* vals = reinterpret_cast<const T*>(g.csc_val);
         for (int jcol = threadIdx.x; jcol < n; jcol += blockDim.x) {
@@ -211,6 +220,10 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
 
       // drift + increment + update (TWMCBaseSolver.cpp:245-253)
       const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
+      // zero the compiler cannot see through: the sites' neighbour addresses are
+      // formed in each step instead of being hoisted and held across the loop
+      int opq = 0;
+      if constexpr (NN) asm volatile("" : "+s"(opq));
       // F(t) before this step (TWMCLiebSolver.cpp:252-256)
       T fdt = T(0);
       const cpx<T>* fv = Fp;
@@ -230,11 +243,11 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
           box_muller<T>(x.z, x.w, xr[1], xi[1]);
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
-            int i = 2 * p + h;
+            int i = 2 * p + h + opq;
             if (i < n) {
               cpx<T> z = beta[2 * j + h];
               cpx<T> hh = cur[i];
-              if (g.hopping == 0) {
+              if (NN || g.hopping == 0) {
                 if constexpr (kStencil) {
                   // h = -i J S, S the sum over the periodic nearest neighbours
                   // (a dimension of length 1 has none): IDFT(K DFT beta)/N exactly
@@ -481,7 +494,39 @@ hipError_t launch_generic_p(const StepArgs& a, int grid, int block, size_t lds, 
   }
 }
 
+template <typename T>
+const void* generic_nn_ptr(int ppt) {
+  return ppt == 1   ? (const void*)generic_step_kernel<T, 1, true>
+         : ppt == 2 ? (const void*)generic_step_kernel<T, 2, true>
+         : ppt == 8 ? (const void*)generic_step_kernel<T, 8, true>
+                    : (const void*)generic_step_kernel<T, 32, true>;
+}
+
 }  // namespace
+
+// LDS of a nearest-neighbour step launch: the per-wave sums and one copy of the state
+size_t generic_nn_lds_bytes(int n, int precision) { return 64 + (size_t)n * (precision == 64 ? 16 : 8); }
+
+hipError_t generic_nn_occupancy(int precision, int n, size_t lds, int* blocks_per_cu) {
+  const int ppt = generic_pairs_per_thread(n, kBlock);
+  const void* f = precision == 64 ? generic_nn_ptr<double>(ppt) : generic_nn_ptr<float>(ppt);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, kBlock, lds);
+}
+
+hipError_t launch_generic_nn(const StepArgs& a, int precision, int grid, size_t lds, hipStream_t s) {
+  const int ppt = generic_pairs_per_thread(a.n, kBlock);
+  const void* f = precision == 64 ? generic_nn_ptr<double>(ppt) : generic_nn_ptr<float>(ppt);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  void* args[] = {const_cast<StepArgs*>(&a)};
+  return hipLaunchKernel(f, dim3(grid), dim3(kBlock), args, lds, s);
+}
 
 int generic_pairs_per_thread(int n, int block) {
   int np = (n + 1) / 2;

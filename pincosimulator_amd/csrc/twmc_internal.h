@@ -174,6 +174,11 @@ hipError_t launch_generic(const StepArgs& a, int precision, int grid, int block,
                           hipStream_t s);
 size_t generic_lds_bytes(int nx, int ny, int precision);
 int generic_pairs_per_thread(int n, int block);
+// the LDS kernel as a step launch with the hopping in its nearest-neighbour
+// form and no capture (FFT hopping): one LDS copy of the state
+size_t generic_nn_lds_bytes(int n, int precision);
+hipError_t generic_nn_occupancy(int precision, int n, size_t lds, int* blocks_per_cu);
+hipError_t launch_generic_nn(const StepArgs& a, int precision, int grid, size_t lds, hipStream_t s);
 hipError_t generic_occupancy(int precision, int n, int block, size_t lds, int* blocks_per_cu);
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
 
@@ -223,6 +228,7 @@ hipError_t mfma_occupancy(int np, int* blocks_per_cu);
 // splits A (row-major np x np, FP64) into BF16 hi and lo parts (raw 16-bit)
 void mfma_split(const double* a, size_t count, uint16_t* hi, uint16_t* lo);
 int mfma_rows(int np);  // rows of the zero-padded A operand
+int mfma_block(int np);  // threads per workgroup of the MFMA step kernel
 void mfma_swizzle(const double* a, int rows, int np, double* out);  // row-major -> fragment order
 
 }  // namespace twmc

@@ -468,6 +468,341 @@ __global__ void __launch_bounds__(256, 1) mfma_step_kernel(StepArgs g) {
   }
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// The step on v_mfma_f32_16x16x32_bf16 for np = 128 TPW (np = 128, 256,
+// 384): 8 waves (two per SIMD, so one wave's MFMAs run while the other waits
+// for operands or at a barrier), 32 trajectories per workgroup.  Wave w owns
+// the TPW 16-site row tiles w TPW .. w TPW + TPW - 1 and all four 16-column
+// tiles: Re and Im of trajectories 0-15 (ct = 0, 1) and 16-31 (ct = 2, 3).
+// Lane l holds column l % 16 and rows 4 (l / 16) + r, r < 4, of each result
+// tile, so Re and Im of the same (site, trajectory) meet in one lane.  A
+// (16-row tiles, 32-wide k blocks, fragment order) streams from L2 one k block
+// ahead; B (the state as BF16 hi/lo) is staged in LDS every step.  A lane owns
+// 4 TPW site pairs per step, one per k block: each k block's MFMAs carry that
+// pair's Philox draw, Box-Muller and drift/drive terms (fenced pieces); after
+// the GEMM only dt h is added.
+template <int TPW, bool CAPTURE_ONLY>
+__global__ void __launch_bounds__(512, 1) mfma16_step_kernel(StepArgs g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NP = 128 * TPW, NKB = NP / 32, LDK = NP + 8;  // LDK: padded row, conflict-free reads
+  const int n = g.n;
+  uint16_t* Xhi = reinterpret_cast<uint16_t*>(smem);
+  uint16_t* Xlo = Xhi + 64 * LDK;
+  cpx<float>* Prsl = reinterpret_cast<cpx<float>*>(Xlo + 64 * LDK);
+  cpx<float>* PU = Prsl + NP;
+  cpx<float>* PF = PU + NP;
+  double* red = reinterpret_cast<double*>(PF + NP);  // [8 waves][32 traj][2]
+
+  // the wave index is uniform: made scalar, the operand addresses become a
+  // scalar base plus one per-lane offset instead of a register pair per load
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int col = lane & 15, rg = lane >> 4;
+  const float dt = float(g.dt), sig = float(g.dt4);
+  const float nl = -1.3862943611198906f * sig * sig;  // -2 ln2 sigma^2 (Box-Muller)
+  // site tables pre-scaled by dt, as in mfma_step_kernel
+  for (int i = tid; i < NP; i += blockDim.x) {
+    const bool in = i < n;
+    const cpx<float> r = in ? reinterpret_cast<const cpx<float>*>(g.rsl)[i] : mk(0.f, 0.f);
+    const cpx<float> u = in ? reinterpret_cast<const cpx<float>*>(g.U)[i] : mk(0.f, 0.f);
+    const cpx<float> f = in ? reinterpret_cast<const cpx<float>*>(g.F)[i] : mk(0.f, 0.f);
+    Prsl[i] = mk(dt * (r.re + u.im), dt * (r.im - u.re));
+    PU[i] = mk(dt * u.re, dt * u.im);
+    PF[i] = mk(-dt * f.im, dt * f.re);
+  }
+  __syncthreads();
+
+  const uint16_t* Ahi = reinterpret_cast<const uint16_t*>(g.a_hi);
+  const uint16_t* Alo = reinterpret_cast<const uint16_t*>(g.a_lo);
+  double* part = g.partials + (size_t)blockIdx.x * kQ * n;
+  const int nblk = (g.n_traj + kTB - 1) / kTB;
+
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    int jt[2];
+    bool valid[2];
+    uint64_t traj[2];
+    cpx<float>* gb[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      jt[q] = blk * kTB + q * 16 + col;
+      valid[q] = jt[q] < g.n_traj;
+      traj[q] = valid[q] ? g.traj_ids[jt[q]] : 0;
+      gb[q] = reinterpret_cast<cpx<float>*>(g.beta) + (size_t)(valid[q] ? jt[q] : 0) * n;
+    }
+    // first site of this lane's rows, opaque to the compiler (per-site
+    // addresses are formed in the block, not held across the steps)
+    int ib = wave * TPW * 16 + rg * 4;
+    asm volatile("" : "+v"(ib));
+
+    // FP32 state in the result layout: [tile][trajectory group][register]
+    float sre[TPW][2][4], sim[TPW][2][4];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t)
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = ib + t * 16 + r;
+          const cpx<float> v = (valid[q] && i < n) ? gb[q][i] : mk(0.f, 0.f);
+          sre[t][q][r] = v.re;
+          sim[t][q][r] = v.im;
+        }
+
+    if (g.capture) {
+      double m2[2] = {0.0, 0.0}, m4[2] = {0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = ib + t * 16 + r;
+          double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const bool use = valid[q] && i < n;
+            const double xr = use ? double(sre[t][q][r]) : 0.0, xi = use ? double(sim[t][q][r]) : 0.0;
+            const double nn = xr * xr + xi * xi;
+            m2[q] += nn;
+            m4[q] += nn * nn;
+            if (g.dump && use)
+              reinterpret_cast<cpx<float>*>(g.traj_out)[((size_t)jt[q] * g.n_frames + g.frame) * n + i] =
+                  mk(sre[t][q][r], sim[t][q][r]);
+            a += nn;
+            b += nn * nn;
+            c += xr;
+            d += xi;
+          }
+          // per-site sums over the block's 32 trajectories: the 16 lanes of a
+          // row group hold the same site for trajectories q 16 + l % 16
+#pragma unroll
+          for (int o = 1; o < 16; o <<= 1) {
+            a += __shfl_xor(a, o, 64);
+            b += __shfl_xor(b, o, 64);
+            c += __shfl_xor(c, o, 64);
+            d += __shfl_xor(d, o, 64);
+          }
+          if (col == 0 && i < n) {
+            part[i] += a;
+            part[n + i] += b;
+            part[3 * n + i] += c;
+            part[4 * n + i] += d;
+          }
+        }
+      // the four row groups hold the same trajectories
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        m2[q] += __shfl_xor(m2[q], 16, 64);
+        m2[q] += __shfl_xor(m2[q], 32, 64);
+        m4[q] += __shfl_xor(m4[q], 16, 64);
+        m4[q] += __shfl_xor(m4[q], 32, 64);
+        if (rg == 0) {
+          red[(wave * 32 + q * 16 + col) * 2] = m2[q];
+          red[(wave * 32 + q * 16 + col) * 2 + 1] = m4[q];
+        }
+      }
+      __syncthreads();
+      if (tid < 32 && blk * kTB + tid < g.n_traj) {
+        double a2 = 0.0, a4 = 0.0;
+        for (int w = 0; w < 8; ++w) {
+          a2 += red[(w * 32 + tid) * 2];
+          a4 += red[(w * 32 + tid) * 2 + 1];
+        }
+        const size_t o = (size_t)(blk * kTB + tid) * g.n_frames + g.frame;
+        g.traj_m2[o] = a2 / double(n);
+        g.traj_m4[o] = a4 / double(n);
+      }
+      __syncthreads();
+    }
+    if (CAPTURE_ONLY) continue;
+
+    int ft_id = 0;
+    for (int s = 0; s < g.n_steps; ++s) {
+      // stage the state as the BF16 hi/lo B operand: columns q 16 + l % 16 (Re)
+      // and 32 + q 16 + l % 16 (Im), this lane's four consecutive sites
+#pragma unroll
+      for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int k0 = (wave * TPW + t) * 16 + rg * 4;
+          uint32_t rh[2], rl[2], ih[2], il[2];
+#pragma unroll
+          for (int e = 0; e < 2; ++e) {
+            split_bf16(sre[t][q][2 * e], sre[t][q][2 * e + 1], rh[e], rl[e]);
+            split_bf16(sim[t][q][2 * e], sim[t][q][2 * e + 1], ih[e], il[e]);
+          }
+          const int cr = q * 16 + col, ci = 32 + q * 16 + col;
+          *reinterpret_cast<uint2*>(&Xhi[cr * LDK + k0]) = make_uint2(rh[0], rh[1]);
+          *reinterpret_cast<uint2*>(&Xlo[cr * LDK + k0]) = make_uint2(rl[0], rl[1]);
+          *reinterpret_cast<uint2*>(&Xhi[ci * LDK + k0]) = make_uint2(ih[0], ih[1]);
+          *reinterpret_cast<uint2*>(&Xlo[ci * LDK + k0]) = make_uint2(il[0], il[1]);
+        }
+      if (g.ft_n > 0) {
+        // F(t) before this step into the site table (TWMCLiebSolver.cpp:252-256),
+        // read during the GEMM after the barrier below; the previous step's
+        // readers finished before the barrier that ended its GEMM
+        const float fdt = float(drive_interval(g.ft_times, g.ft_n, g.t0 + double(s) * g.dt, ft_id));
+        const cpx<float>* fv = reinterpret_cast<const cpx<float>*>(g.ft_vals) + (size_t)ft_id * n;
+        const cpx<float>* fs = reinterpret_cast<const cpx<float>*>(g.ft_slopes) + (size_t)ft_id * n;
+        for (int i = tid; i < n; i += blockDim.x)
+          PF[i] = mk(-dt * fmaf(fdt, fs[i].im, fv[i].im), dt * fmaf(fdt, fs[i].re, fv[i].re));
+      }
+      __syncthreads();
+
+      constexpr int kPieces = 9, kSlots = 12 * TPW;
+      const uint32_t step = uint32_t(g.step0 + (uint64_t)s);
+      // opaque to the compiler: the per-pair Philox rounds, the table reads and
+      // the operand addresses are formed in the step, not hoisted out of it
+      uint32_t pbase = (uint32_t)(wave * TPW * 8 + rg * 2);
+      asm volatile("" : "+v"(pbase));
+      int zo = 0;
+      asm volatile("" : "+s"(zo));
+      const uint16_t* Ahw = Ahi + (size_t)(wave * TPW) * NKB * 512 + zo;  // scalar
+      const uint16_t* Alw = Alo + (size_t)(wave * TPW) * NKB * 512 + zo;
+      auto aload = [&](const uint16_t* A, int t, int kb) {
+        return *reinterpret_cast<const bf16x8*>(&(A == Ahi ? Ahw : Alw)[((size_t)t * NKB + kb) * 512 + lane * 8]);
+      };
+      auto bload = [&](const uint16_t* X, int ct, int kb) {
+        const int c = (ct & 1) * 32 + (ct >> 1) * 16 + col;
+        return *reinterpret_cast<const bf16x8*>(&X[c * LDK + kb * 32 + rg * 8 + zo]);
+      };
+      f32x4 acc[TPW][4];
+#pragma unroll
+      for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[t][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+      bf16x8 ah[TPW], al[TPW], bh, bl;
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) {
+        ah[t] = aload(Ahi, t, 0);
+        al[t] = aload(Alo, t, 0);
+      }
+      bh = bload(Xhi, 0, 0);
+      bl = bload(Xlo, 0, 0);
+#pragma unroll
+      for (int kb = 0; kb < NKB; ++kb) {
+        // A of the next k block, one block ahead
+        bf16x8 ahn[TPW], aln[TPW];
+        if (kb + 1 < NKB) {
+#pragma unroll
+          for (int t = 0; t < TPW; ++t) {
+            ahn[t] = aload(Ahi, t, kb + 1);
+            aln[t] = aload(Alo, t, kb + 1);
+          }
+        }
+        // site pair kb: row tile tp, trajectory group qp, pair pr of the lane's four sites
+        const int tp = kb >> 2, qp = (kb >> 1) & 1, pr = kb & 1;
+        u32x4 w;
+        float rn[2], cs[2], sn[2];
+        auto piece = [&](int pc) __attribute__((always_inline)) {
+          if (pc == 0) w = make_counter(traj[qp], TAG_WIENER, step, pbase + (uint32_t)(tp * 8 + pr));
+          if (pc < 5) {
+            philox_round(w, g.key0, g.key1, 2 * pc);
+            philox_round(w, g.key0, g.key1, 2 * pc + 1);
+            asm volatile("" : "+v"(w.x), "+v"(w.y), "+v"(w.z), "+v"(w.w));
+          } else if (pc < 7) {
+            const int h = pc - 5;
+            box_muller_f32_scaled(h ? w.z : w.x, h ? w.w : w.y, nl, rn[h], cs[h], sn[h]);
+            asm volatile("" : "+v"(rn[h]), "+v"(cs[h]), "+v"(sn[h]));
+          } else {
+            // increment without dt h: noise + dt iF + drift (TWMCLiebSolver.cpp:259-264)
+            const int h = pc - 7, r = 2 * pr + h;
+            const int i = (wave * TPW + tp) * 16 + rg * 4 + r;
+            const float br = sre[tp][qp][r], bi = sim[tp][qp][r];
+            const cpx<float> ab = Prsl[i + zo], u = PU[i + zo], f = PF[i + zo];
+            const float mq = fmaf(bi, bi, br * br);
+            const float cr = fmaf(-u.im, mq, ab.re), ci = fmaf(u.re, mq, ab.im);
+            float dr = fmaf(rn[h], cs[h], f.re);
+            dr = fmaf(cr, br, dr);
+            dr = fmaf(-ci, bi, dr);
+            float di = fmaf(rn[h], sn[h], f.im);
+            di = fmaf(cr, bi, di);
+            di = fmaf(ci, br, di);
+            // added to the state now (the B operand of this step is already
+            // staged); dt h follows after the GEMM
+            sre[tp][qp][r] = br + dr;
+            sim[tp][qp][r] = bi + di;
+            asm volatile("" : "+v"(sre[tp][qp][r]), "+v"(sim[tp][qp][r]));
+          }
+        };
+        // 12 TPW MFMAs: column tile ct outer (its B operand loaded one tile
+        // ahead), then the products hi.hi, hi.lo, lo.hi, then the row tiles
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+          bf16x8 bhn = bh, bln = bl;
+          if (ct + 1 < 4) {
+            bhn = bload(Xhi, ct + 1, kb);
+            bln = bload(Xlo, ct + 1, kb);
+          } else if (kb + 1 < NKB) {
+            bhn = bload(Xhi, 0, kb + 1);
+            bln = bload(Xlo, 0, kb + 1);
+          }
+#pragma unroll
+          for (int pm = 0; pm < 3; ++pm)
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+              const int sl = (ct * 3 + pm) * TPW + t;
+#pragma unroll
+              for (int pc = (sl * kPieces + kSlots - 1) / kSlots; pc < ((sl + 1) * kPieces + kSlots - 1) / kSlots;
+                   ++pc)
+                piece(pc);
+              const bf16x8 a = pm < 2 ? ah[t] : al[t];
+              const bf16x8 b = pm == 1 ? bl : bh;
+              acc[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t][ct], 0, 0, 0);
+              __builtin_amdgcn_sched_barrier(0);
+            }
+          bh = bhn;
+          bl = bln;
+        }
+        if (kb + 1 < NKB) {
+#pragma unroll
+          for (int t = 0; t < TPW; ++t) {
+            ah[t] = ahn[t];
+            al[t] = aln[t];
+          }
+        }
+      }
+      __syncthreads();  // the next step overwrites the B operand
+      // dt h, h = -i Y (h_re = Im Y, h_im = -Re Y), and the update
+#pragma unroll
+      for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = (wave * TPW + t) * 16 + rg * 4 + r + zo;
+            if (i < n) {
+              sre[t][q][r] = fmaf(dt, acc[t][2 * q + 1][r], sre[t][q][r]);
+              sim[t][q][r] = fmaf(-dt, acc[t][2 * q][r], sim[t][q][r]);
+            }
+          }
+    }
+
+    if (!CAPTURE_ONLY) {
+#pragma unroll
+      for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = ib + t * 16 + r;
+            if (valid[q] && i < n) gb[q][i] = mk(sre[t][q][r], sim[t][q][r]);
+          }
+    }
+  }
+}
+
+template <int TPW>
+hipError_t launch16_t(const StepArgs& a, int grid, size_t lds, hipStream_t s) {
+  const void* f = a.n_steps > 0 ? (const void*)mfma16_step_kernel<TPW, false>
+                                : (const void*)mfma16_step_kernel<TPW, true>;
+  hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  if (a.n_steps > 0)
+    hipLaunchKernelGGL((mfma16_step_kernel<TPW, false>), dim3(grid), dim3(512), lds, s, a);
+  else
+    hipLaunchKernelGGL((mfma16_step_kernel<TPW, true>), dim3(grid), dim3(512), lds, s, a);
+  return hipGetLastError();
+}
+
 template <int TPW, bool OVL>
 hipError_t launch_t(const StepArgs& a, int grid, size_t lds, hipStream_t s) {
   const void* f = a.n_steps > 0 ? (const void*)mfma_step_kernel<TPW, OVL, false>
@@ -493,23 +828,34 @@ void mfma_shape(int np, int* tpw, bool* ovl) {
 
 int mfma_padded(int n) { return (n + 31) / 32 * 32; }
 
-// rows of the padded A operand: 4 waves x TPW tiles x 32
-int mfma_rows(int np) { return 4 * ((np / 32 + 3) / 4) * 32; }
+// the 16x16x32 step (8 waves) for np = 128, 256, 384; 32x32x16 otherwise
+// (at np = 512 the 16x16x32 step's four row tiles per wave spill registers)
+bool mfma_use16(int np) { return np % 128 == 0 && np <= 384; }
 
-// row-major rows x np -> fragment order: for row tile T, 16-wide k block b
-// and lane l, 8 consecutive elements A[32 T + l % 32][16 b + 8 (l / 32) + e]
+// rows of the padded A operand: 8 waves x TPW tiles x 16, or 4 waves x TPW tiles x 32
+int mfma_rows(int np) { return mfma_use16(np) ? np : 4 * ((np / 32 + 3) / 4) * 32; }
+
+// row-major rows x np -> fragment order.  16x16x32: for 16-row tile T, 32-wide
+// k block b and lane l, the 8 consecutive elements A[16 T + l % 16][32 b +
+// 8 (l / 16) + e]; 32x32x16: for 32-row tile T, 16-wide k block b, the 8
+// elements A[32 T + l % 32][16 b + 8 (l / 32) + e]
 void mfma_swizzle(const double* a, int rows, int np, double* out) {
-  const int nkb = np / 16;
-  for (int T = 0; T < rows / 32; ++T)
+  const bool w16 = mfma_use16(np);
+  const int tr = w16 ? 16 : 32, kw = w16 ? 32 : 16, nkb = np / kw;
+  for (int T = 0; T < rows / tr; ++T)
     for (int b = 0; b < nkb; ++b)
       for (int l = 0; l < 64; ++l)
         for (int e = 0; e < 8; ++e)
-          out[(((size_t)T * nkb + b) * 64 + l) * 8 + e] = a[(size_t)(T * 32 + l % 32) * np + b * 16 + (l / 32) * 8 + e];
+          out[(((size_t)T * nkb + b) * 64 + l) * 8 + e] =
+              a[(size_t)(T * tr + l % tr) * np + b * kw + (l / tr) * 8 + e];
 }
 
 size_t mfma_lds_bytes(int np) {
-  return (size_t)2 * 64 * (np + 8) * 2 + (size_t)3 * np * 8 + 4 * 32 * 2 * 8;
+  const int waves = mfma_use16(np) ? 8 : 4;
+  return (size_t)2 * 64 * (np + 8) * 2 + (size_t)3 * np * 8 + (size_t)waves * 32 * 2 * 8;
 }
+
+int mfma_block(int np) { return mfma_use16(np) ? 512 : 256; }
 
 bool mfma_supported(int n, int hopping, int precision, int param_stride) {
   int np = mfma_padded(n);
@@ -517,10 +863,17 @@ bool mfma_supported(int n, int hopping, int precision, int param_stride) {
 }
 
 hipError_t launch_mfma(const StepArgs& a, int grid, hipStream_t s) {
+  const size_t lds = mfma_lds_bytes(a.np);
+  if (mfma_use16(a.np)) {
+    switch (a.np / 128) {
+      case 1: return launch16_t<1>(a, grid, lds, s);
+      case 2: return launch16_t<2>(a, grid, lds, s);
+      default: return launch16_t<3>(a, grid, lds, s);
+    }
+  }
   int tpw;
   bool ovl;
   mfma_shape(a.np, &tpw, &ovl);
-  const size_t lds = mfma_lds_bytes(a.np);
   switch (tpw) {
     case 1: return ovl ? launch_t<1, true>(a, grid, lds, s) : launch_t<1, false>(a, grid, lds, s);
     case 2: return ovl ? launch_t<2, true>(a, grid, lds, s) : launch_t<2, false>(a, grid, lds, s);
@@ -531,6 +884,15 @@ hipError_t launch_mfma(const StepArgs& a, int grid, hipStream_t s) {
 
 hipError_t mfma_occupancy(int np, int* blocks_per_cu) {
   const size_t lds = mfma_lds_bytes(np);
+  if (mfma_use16(np)) {
+    const int t = np / 128;
+    const void* f = t == 1   ? (const void*)mfma16_step_kernel<1, false>
+                    : t == 2 ? (const void*)mfma16_step_kernel<2, false>
+                             : (const void*)mfma16_step_kernel<3, false>;
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 512, lds);
+  }
   int tpw;
   bool ovl;
   mfma_shape(np, &tpw, &ovl);
