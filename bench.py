@@ -280,6 +280,19 @@ def measure_engine(base, n_traj, fsteps, kernel, precision, W, K):
     return site_steps / wall, st["step_kernel_ms"] / max(st["step_launches"], 1), info
 
 
+def profile_fields(name):
+    """traffic and the limiter's counters from a committed rocprofv3 summary
+    of the same kernel and workload (profiles/<name>), or nothing."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        prof = json.load(open(path))
+    except (OSError, ValueError):
+        return {}
+    pmc = {k: prof[k] for k in ("valu_insts_per_site_step", "valu_issue_utilisation", "mfma_util") if k in prof}
+    pmc["source"] = os.path.relpath(path, ROOT)
+    return {"traffic": prof.get("hbm_bytes_per_launch"), "pmc": pmc}
+
+
 def secondary_lines():
     """The other measured rows, in the default line: the 64x64 workload in FP64
     (the reference's precision), the Lieb chain on MFMA and on the banded
@@ -290,11 +303,13 @@ def secondary_lines():
     log(f"bench: FP64 64x64: {v:.4e} site-steps/s, step kernel avg {ms:.2f} ms")
     out["fp64"] = {
         "value": v, "unit": "trajectory·site-steps/s", "dtype": "f64", "kernel": info["kernel"] + "_step_kernel",
-        "workload": "the 64x64 workload above (16384 trajectories, 500-step frames, 2 timed) with an FP64 state, "
-                    "the reference's FFT form every step",
+        "workload": "the 64x64 workload above (16384 trajectories, 500-step frames, 2 timed) with an FP64 state "
+                    "and FP64 Box-Muller; the steps apply the exact nearest-neighbour form of the reference's "
+                    "FFT hopping, the frame captures the FFT",
         "roofline": {"bound": "valu", "achieved": a / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": a / PEAK_FP64, "avg_launch_ms": ms,
-                     "flops_basis": "algorithmic: 150 flops per site-step, as the line's roofline"}}
+                     "flops_basis": "algorithmic: 150 flops per site-step, as the line's roofline",
+                     **profile_fields("generic_fp64_64x64_16384traj_pmc.json")}}
     lieb = lieb_spec()
     n = lieb["nx"] * lieb["ny"]
     v, ms, _ = measure_engine(lieb, LIEB_TRAJ, LIEB_FRAME, "mfma", 32, 1, 10)
@@ -303,10 +318,11 @@ def secondary_lines():
     log(f"bench: lieb384: MFMA {v:.4e} site-steps/s ({ms:.3f} ms per launch), band {vb:.4e} ({msb:.3f} ms)")
     out["lieb384"] = {
         "workload": WORKLOAD_TEXT["lieb384"] + "; 10 timed frames",
-        "mfma": {"value": v, "unit": "trajectory·site-steps/s", "kernel": "mfma_step_kernel", "avg_launch_ms": ms,
+        "mfma": {"value": v, "unit": "trajectory·site-steps/s", "kernel": "mfma16_step_kernel", "avg_launch_ms": ms,
                  "roofline": {"bound": "mfma", "achieved": a / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
                               "frac": a / PEAK_BF16,
-                              "flops_basis": "executed BF16 MFMA flops: 3 products x 4 N per site-step"}},
+                              "flops_basis": "executed BF16 MFMA flops: 3 products x 4 N per site-step",
+                              **profile_fields("mfma_lieb384_8192traj_pmc.json")}},
         "band": {"value": vb, "unit": "trajectory·site-steps/s", "kernel": "band_step_kernel", "avg_launch_ms": msb}}
     import math
     from pincosimulator_amd import Sweep
@@ -402,7 +418,7 @@ def main():
     if args.workload == "lieb384" and info["kernel"] == "mfma":
         # executed MFMA work: 3 BF16 products of the real N x N hopping on Re and Im
         flops_site = 3 * 4.0 * n_sites
-        peak, bound, kname = PEAK_BF16, "mfma", "mfma_step_kernel"
+        peak, bound, kname = PEAK_BF16, "mfma", "mfma16_step_kernel"
         basis = "executed BF16 MFMA flops: 3 products (hi.hi, hi.lo, lo.hi) x 4 N per site-step"
     elif args.workload == "lieb384":
         # the banded kernel: executed FP32 arithmetic per site-step, the
