@@ -1,0 +1,170 @@
+// twmc_f64math.h -- FP64 Box-Muller for the per-step Wiener increments of the
+// FP64 kernels, written for the arguments Box-Muller takes here: u1 = (a + 1/2)
+// 2^-32 and theta = b 2^-32 revolutions for 32-bit integers a, b.  General
+// library log / sin / cos spend most of their instructions on argument ranges
+// that never occur here (large angles, subnormals, infinities); these forms
+// reduce exactly in integer arithmetic and evaluate short series:
+//   sin, cos: the quarter turn nearest to b is removed from b itself, leaving
+//     an angle |x| <= pi/4, then the Taylor series through x^17 (sin) and x^16
+//     (cos), truncated below 1e-19;
+//   ln: u1 = m 2^e with m in (sqrt(1/2), sqrt(2)] from the exponent bits,
+//     ln m = 2 atanh(f), f = (m - 1)/(m + 1), |f| <= 0.1716, as its series
+//     through f^21 (truncated below 3e-18 relative), plus e ln 2 with ln 2
+//     split so that e ln2_hi is exact.
+// N samples are evaluated side by side, so each series coefficient is formed
+// once for all N.  Every multiply-add is an explicit fma and contraction is
+// off, so the host build (used by the accuracy test) evaluates the same
+// correctly rounded operations in the same order as the device build.  Pure
+// C++: no HIP header is needed to compile it on the host.
+#ifndef TWMC_F64MATH_H
+#define TWMC_F64MATH_H
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__HIPCC__)
+#define TWMC_HD __host__ __device__
+#else
+#define TWMC_HD
+#endif
+#if defined(__clang__)
+#define TWMC_NO_CONTRACT _Pragma("clang fp contract(off)")
+#define TWMC_UNROLL _Pragma("unroll")
+#else
+#define TWMC_NO_CONTRACT
+#define TWMC_UNROLL _Pragma("GCC unroll 16")
+#endif
+
+namespace twmc {
+namespace f64 {
+
+// 2 pi 2^-32: the double nearest 2 pi, scaled by a power of two
+constexpr double kTurn32 = 6.283185307179586 / 4294967296.0;
+// ln 2 = kLn2Hi + kLn2Lo to 2e-31; kLn2Hi has 44 significant bits
+constexpr double kLn2Hi = 0.6931471805599472;
+constexpr double kLn2Lo = -1.8641886737243033e-15;
+
+// a b + c as one three-operand v_fma_f64 on the GPU.  The series' steps
+// p = p z + k read a coefficient k that stays live for the next site; as the
+// two-operand v_fmac_f64 the compiler would first copy k into the result
+// register, one more instruction per step.
+TWMC_HD inline double fma3(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+#else
+  return fma(a, b, c);
+#endif
+}
+
+TWMC_HD inline uint64_t bits(double x) {
+  uint64_t u;
+  memcpy(&u, &x, sizeof u);
+  return u;
+}
+
+TWMC_HD inline double from_bits(uint64_t u) {
+  double x;
+  memcpy(&x, &u, sizeof x);
+  return x;
+}
+
+// sin and cos of 2 pi b[i] 2^-32, i < N.
+template <int N>
+TWMC_HD inline void sincos_turn32(const uint32_t* b, double* s, double* c) {
+  TWMC_NO_CONTRACT
+  double x[N], z[N], ps[N], pc[N];
+  uint32_t q[N];
+  TWMC_UNROLL
+  for (int i = 0; i < N; ++i) {
+    // nearest quarter turn q (mod 4) and the remainder b - q 2^30 in [-2^29, 2^29)
+    q[i] = (b[i] + 0x20000000u) >> 30;
+    const int32_t r = int32_t(b[i] - (q[i] << 30));
+    x[i] = double(r) * kTurn32;  // |x| <= pi/4
+    z[i] = x[i] * x[i];
+    ps[i] = -1.0 / 355687428096000.0;  // -1/17!
+    pc[i] = 1.0 / 20922789888000.0;    // 1/16!
+  }
+  // sin x = x - x z (1/3! - z/5! + ...), cos x = 1 + z (-1/2! + z/4! - ...), z = x^2
+  const double ks[7] = {1.0 / 1307674368000.0, -1.0 / 6227020800.0, 1.0 / 39916800.0, -1.0 / 362880.0,
+                        1.0 / 5040.0,          -1.0 / 120.0,        1.0 / 6.0};
+  const double kc[7] = {-1.0 / 87178291200.0, 1.0 / 479001600.0, -1.0 / 3628800.0, 1.0 / 40320.0,
+                        -1.0 / 720.0,         1.0 / 24.0,        -0.5};
+  TWMC_UNROLL
+  for (int j = 0; j < 7; ++j)
+    TWMC_UNROLL
+    for (int i = 0; i < N; ++i) {
+      ps[i] = fma3(ps[i], z[i], ks[j]);
+      pc[i] = fma3(pc[i], z[i], kc[j]);
+    }
+  TWMC_UNROLL
+  for (int i = 0; i < N; ++i) {
+    const double sx = fma(-(x[i] * z[i]), ps[i], x[i]);
+    const double cx = fma(pc[i], z[i], 1.0);
+    // rotate by q quarter turns: (c, s) -> (-s, c) per quarter
+    const bool swap = (q[i] & 1u) != 0;
+    const double cs = swap ? sx : cx, sn = swap ? cx : sx;
+    c[i] = ((q[i] + 1u) & 2u) ? -cs : cs;
+    s[i] = (q[i] & 2u) ? -sn : sn;
+  }
+}
+
+// ln((a[i] + 1/2) 2^-32), in (-22.9, 0), i < N.
+template <int N>
+TWMC_HD inline void log_u32(const uint32_t* a, double* out) {
+  TWMC_NO_CONTRACT
+  double f[N], w[N], p[N];
+  int e[N];
+  TWMC_UNROLL
+  for (int i = 0; i < N; ++i) {
+    const double u = (double(a[i]) + 0.5) * (1.0 / 4294967296.0);  // exact, normal
+    const uint64_t ub = bits(u);
+    // u = m 2^e with m in [1, 2), then m in (sqrt(1/2), sqrt(2)]
+    e[i] = int((ub >> 52) & 0x7ff) - 1023;
+    uint64_t mb = (ub & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    if (mb > 0x3ff6a09e667f3bcdull) {  // m > sqrt(2) (the double nearest it)
+      mb -= 0x0010000000000000ull;     // m / 2, exactly
+      ++e[i];
+    }
+    const double m = from_bits(mb);
+    f[i] = (m - 1.0) / (m + 1.0);  // m - 1 exact
+    w[i] = f[i] * f[i];
+    p[i] = 2.0 / 21.0;
+  }
+  // ln m = 2 f + f w (2/3 + 2 w/5 + ... + 2 w^9/21)
+  const double kl[9] = {2.0 / 19.0, 2.0 / 17.0, 2.0 / 15.0, 2.0 / 13.0, 2.0 / 11.0,
+                        2.0 / 9.0,  2.0 / 7.0,  2.0 / 5.0,  2.0 / 3.0};
+  TWMC_UNROLL
+  for (int j = 0; j < 9; ++j)
+    TWMC_UNROLL
+    for (int i = 0; i < N; ++i) p[i] = fma3(p[i], w[i], kl[j]);
+  TWMC_UNROLL
+  for (int i = 0; i < N; ++i) {
+    const double lnm = fma(f[i] * w[i], p[i], f[i] + f[i]);
+    const double de = double(e[i]);
+    out[i] = fma(de, kLn2Hi, fma(de, kLn2Lo, lnm));
+  }
+}
+
+// N complex N(0, 1) samples from the Philox word pairs (a[i], b[i]):
+// Re = sqrt(-2 ln u1) cos(2 pi theta), Im = sqrt(-2 ln u1) sin(2 pi theta).
+template <int N>
+TWMC_HD inline void box_muller(const uint32_t* a, const uint32_t* b, double* re, double* im) {
+  TWMC_NO_CONTRACT
+  double l[N], s[N], c[N];
+  log_u32<N>(a, l);
+  sincos_turn32<N>(b, s, c);
+  TWMC_UNROLL
+  for (int i = 0; i < N; ++i) {
+    const double r = sqrt(-2.0 * l[i]);
+    re[i] = r * c[i];
+    im[i] = r * s[i];
+  }
+}
+
+}  // namespace f64
+}  // namespace twmc
+
+#endif

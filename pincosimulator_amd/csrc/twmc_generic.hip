@@ -13,6 +13,7 @@
 #include <algorithm>
 
 #include "twmc_cplx.h"
+#include "twmc_f64math.h"
 #include "twmc_internal.h"
 #include "twmc_rng.h"
 #include "twmc_stockham.h"
@@ -34,12 +35,17 @@ __device__ cpx<T>* fft2d(cpx<T>* a, cpx<T>* b, const StepArgs& g, const cpx<T>* 
   return cur;
 }
 
+// the step's Wiener increments of a site pair from one Philox draw: FP64 in
+// the range-specialised forms of twmc_f64math.h, both sites side by side; FP32
+// on the hardware transcendental units
 template <typename T>
-__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, T& re, T& im) {
+__device__ __forceinline__ void box_muller_pair(const u32x4& x, T (&re)[2], T (&im)[2]) {
   if constexpr (sizeof(T) == 8) {
-    box_muller_f64(a, b, re, im);
+    const uint32_t a[2] = {x.x, x.z}, b[2] = {x.y, x.w};
+    f64::box_muller<2>(a, b, re, im);
   } else {
-    box_muller_f32(a, b, re, im);
+    box_muller_f32(x.x, x.y, re[0], im[0]);
+    box_muller_f32(x.z, x.w, re[1], im[1]);
   }
 }
 
@@ -239,8 +245,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
         if (p < npairs) {
           u32x4 x = philox4x32_10(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
           T xr[2], xi[2];
-          box_muller<T>(x.x, x.y, xr[0], xi[0]);
-          box_muller<T>(x.z, x.w, xr[1], xi[1]);
+          box_muller_pair<T>(x, xr, xi);
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             int i = 2 * p + h + opq;
