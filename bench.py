@@ -469,7 +469,8 @@ def main():
             "workload": WORKLOAD_TEXT[args.workload],
             "lattice": "64x64" if args.workload == "square64" else "Lieb 128x3 OBC",
             "trajectories": n_traj, "euler_steps_per_step": fsteps,
-            "state": "complex64 in registers/LDS, FP64 ensemble sums", "kernel": info["kernel"],
+            "state": ("complex128" if args.precision == 64 else "complex64") + " in registers/LDS, FP64 ensemble sums",
+            "kernel": info["kernel"],
             "grid": st["grid"], "block": st["block"],
         },
         "roofline": {
