@@ -59,6 +59,39 @@ TWMC_HD inline double fma3(double a, double b, double c) {
 #endif
 }
 
+// The series coefficients.  A kernel may hand the functions a copy held in
+// vector registers for its whole run (pin), instead of having every use form
+// its constant anew.
+struct Series {
+  double s_top;  // -1/17!
+  double ks[7];  // 1/15!, -1/13!, ..., 1/3!
+  double c_top;  // 1/16!
+  double kc[7];  // -1/14!, 1/12!, ..., -1/2!
+  double l_top;  // 2/21
+  double kl[9];  // 2/19, 2/17, ..., 2/3
+};
+constexpr Series kSeries = {
+    -1.0 / 355687428096000.0,
+    {1.0 / 1307674368000.0, -1.0 / 6227020800.0, 1.0 / 39916800.0, -1.0 / 362880.0, 1.0 / 5040.0, -1.0 / 120.0,
+     1.0 / 6.0},
+    1.0 / 20922789888000.0,
+    {-1.0 / 87178291200.0, 1.0 / 479001600.0, -1.0 / 3628800.0, 1.0 / 40320.0, -1.0 / 720.0, 1.0 / 24.0, -0.5},
+    2.0 / 21.0,
+    {2.0 / 19.0, 2.0 / 17.0, 2.0 / 15.0, 2.0 / 13.0, 2.0 / 11.0, 2.0 / 9.0, 2.0 / 7.0, 2.0 / 5.0, 2.0 / 3.0}};
+
+// On the GPU: makes the coefficients of k values the compiler cannot see
+// through, so it keeps them in vector registers rather than forming each
+// constant at each use.
+TWMC_HD inline void pin(Series& k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(k.s_top), "+v"(k.c_top), "+v"(k.l_top));
+  for (int j = 0; j < 7; ++j) asm volatile("" : "+v"(k.ks[j]), "+v"(k.kc[j]));
+  for (int j = 0; j < 9; ++j) asm volatile("" : "+v"(k.kl[j]));
+#else
+  (void)k;
+#endif
+}
+
 TWMC_HD inline uint64_t bits(double x) {
   uint64_t u;
   memcpy(&u, &x, sizeof u);
@@ -73,7 +106,7 @@ TWMC_HD inline double from_bits(uint64_t u) {
 
 // sin and cos of 2 pi b[i] 2^-32, i < N.
 template <int N>
-TWMC_HD inline void sincos_turn32(const uint32_t* b, double* s, double* c) {
+TWMC_HD inline void sincos_turn32(const uint32_t* b, double* s, double* c, const Series& k = kSeries) {
   TWMC_NO_CONTRACT
   double x[N], z[N], ps[N], pc[N];
   uint32_t q[N];
@@ -84,20 +117,16 @@ TWMC_HD inline void sincos_turn32(const uint32_t* b, double* s, double* c) {
     const int32_t r = int32_t(b[i] - (q[i] << 30));
     x[i] = double(r) * kTurn32;  // |x| <= pi/4
     z[i] = x[i] * x[i];
-    ps[i] = -1.0 / 355687428096000.0;  // -1/17!
-    pc[i] = 1.0 / 20922789888000.0;    // 1/16!
+    ps[i] = k.s_top;
+    pc[i] = k.c_top;
   }
   // sin x = x - x z (1/3! - z/5! + ...), cos x = 1 + z (-1/2! + z/4! - ...), z = x^2
-  const double ks[7] = {1.0 / 1307674368000.0, -1.0 / 6227020800.0, 1.0 / 39916800.0, -1.0 / 362880.0,
-                        1.0 / 5040.0,          -1.0 / 120.0,        1.0 / 6.0};
-  const double kc[7] = {-1.0 / 87178291200.0, 1.0 / 479001600.0, -1.0 / 3628800.0, 1.0 / 40320.0,
-                        -1.0 / 720.0,         1.0 / 24.0,        -0.5};
   TWMC_UNROLL
   for (int j = 0; j < 7; ++j)
     TWMC_UNROLL
     for (int i = 0; i < N; ++i) {
-      ps[i] = fma3(ps[i], z[i], ks[j]);
-      pc[i] = fma3(pc[i], z[i], kc[j]);
+      ps[i] = fma3(ps[i], z[i], k.ks[j]);
+      pc[i] = fma3(pc[i], z[i], k.kc[j]);
     }
   TWMC_UNROLL
   for (int i = 0; i < N; ++i) {
@@ -113,7 +142,7 @@ TWMC_HD inline void sincos_turn32(const uint32_t* b, double* s, double* c) {
 
 // ln((a[i] + 1/2) 2^-32), in (-22.9, 0), i < N.
 template <int N>
-TWMC_HD inline void log_u32(const uint32_t* a, double* out) {
+TWMC_HD inline void log_u32(const uint32_t* a, double* out, const Series& k = kSeries) {
   TWMC_NO_CONTRACT
   double f[N], w[N], p[N];
   int e[N];
@@ -131,15 +160,13 @@ TWMC_HD inline void log_u32(const uint32_t* a, double* out) {
     const double m = from_bits(mb);
     f[i] = (m - 1.0) / (m + 1.0);  // m - 1 exact
     w[i] = f[i] * f[i];
-    p[i] = 2.0 / 21.0;
+    p[i] = k.l_top;
   }
   // ln m = 2 f + f w (2/3 + 2 w/5 + ... + 2 w^9/21)
-  const double kl[9] = {2.0 / 19.0, 2.0 / 17.0, 2.0 / 15.0, 2.0 / 13.0, 2.0 / 11.0,
-                        2.0 / 9.0,  2.0 / 7.0,  2.0 / 5.0,  2.0 / 3.0};
   TWMC_UNROLL
   for (int j = 0; j < 9; ++j)
     TWMC_UNROLL
-    for (int i = 0; i < N; ++i) p[i] = fma3(p[i], w[i], kl[j]);
+    for (int i = 0; i < N; ++i) p[i] = fma3(p[i], w[i], k.kl[j]);
   TWMC_UNROLL
   for (int i = 0; i < N; ++i) {
     const double lnm = fma(f[i] * w[i], p[i], f[i] + f[i]);
@@ -151,11 +178,12 @@ TWMC_HD inline void log_u32(const uint32_t* a, double* out) {
 // N complex N(0, 1) samples from the Philox word pairs (a[i], b[i]):
 // Re = sqrt(-2 ln u1) cos(2 pi theta), Im = sqrt(-2 ln u1) sin(2 pi theta).
 template <int N>
-TWMC_HD inline void box_muller(const uint32_t* a, const uint32_t* b, double* re, double* im) {
+TWMC_HD inline void box_muller(const uint32_t* a, const uint32_t* b, double* re, double* im,
+                               const Series& k = kSeries) {
   TWMC_NO_CONTRACT
   double l[N], s[N], c[N];
-  log_u32<N>(a, l);
-  sincos_turn32<N>(b, s, c);
+  log_u32<N>(a, l, k);
+  sincos_turn32<N>(b, s, c, k);
   TWMC_UNROLL
   for (int i = 0; i < N; ++i) {
     const double r = sqrt(-2.0 * l[i]);

@@ -39,10 +39,11 @@ __device__ cpx<T>* fft2d(cpx<T>* a, cpx<T>* b, const StepArgs& g, const cpx<T>* 
 // the range-specialised forms of twmc_f64math.h, both sites side by side; FP32
 // on the hardware transcendental units
 template <typename T>
-__device__ __forceinline__ void box_muller_pair(const u32x4& x, T (&re)[2], T (&im)[2]) {
+__device__ __forceinline__ void box_muller_pair(const u32x4& x, T (&re)[2], T (&im)[2],
+                                                const f64::Series& k = f64::kSeries) {
   if constexpr (sizeof(T) == 8) {
     const uint32_t a[2] = {x.x, x.z}, b[2] = {x.y, x.w};
-    f64::box_muller<2>(a, b, re, im);
+    f64::box_muller<2>(a, b, re, im, k);
   } else {
     box_muller_f32(x.x, x.y, re[0], im[0]);
     box_muller_f32(x.z, x.w, re[1], im[1]);
@@ -83,6 +84,9 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
   }
 
   const int npairs = (n + 1) / 2;
+  // NN FP64: the Box-Muller series coefficients held in registers for the run
+  f64::Series bmk = f64::kSeries;
+  if constexpr (NN && sizeof(T) == 8) f64::pin(bmk);
   double* part = g.partials + (size_t)blockIdx.x * kQ * n;
   constexpr bool kStencil = sizeof(T) == 4 || NN;
   // x = floor(i / ny) as umulhi(i, ceil(2^32 / ny)): exact for i ny < 2^32
@@ -245,7 +249,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
         if (p < npairs) {
           u32x4 x = philox4x32_10(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
           T xr[2], xi[2];
-          box_muller_pair<T>(x, xr, xi);
+          box_muller_pair<T>(x, xr, xi, bmk);
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             int i = 2 * p + h + opq;
