@@ -3,14 +3,14 @@
 // 2^-32 and theta = b 2^-32 revolutions for 32-bit integers a, b.  General
 // library log / sin / cos spend most of their instructions on argument ranges
 // that never occur here (large angles, subnormals, infinities); these forms
-// reduce exactly in integer arithmetic and evaluate short series:
+// reduce exactly in integer arithmetic and evaluate short polynomials:
 //   sin, cos: the quarter turn nearest to b is removed from b itself, leaving
-//     an angle |x| <= pi/4, then the Taylor series through x^17 (sin) and x^16
-//     (cos), truncated below 1e-19;
+//     an angle |x| <= pi/4; sin x = x - x^3 ps(x^2), cos x = 1 + x^2 pc(x^2);
 //   ln: u1 = m 2^e with m in (sqrt(1/2), sqrt(2)] from the exponent bits,
-//     ln m = 2 atanh(f), f = (m - 1)/(m + 1), |f| <= 0.1716, as its series
-//     through f^21 (truncated below 3e-18 relative), plus e ln 2 with ln 2
-//     split so that e ln2_hi is exact.
+//     ln m = 2 atanh(f) = 2 f + f^3 pl(f^2), f = (m - 1)/(m + 1),
+//     |f| <= 0.1716, plus e ln 2 with ln 2 split so that e ln2_hi is exact;
+//   ps, pc, pl: degree-6 Chebyshev fits over those ranges (gen_f64_series.py),
+//     each adding less than 2e-17 to the relative error of its function.
 // N samples are evaluated side by side, so each series coefficient is formed
 // once for all N.  Every multiply-add is an explicit fma and contraction is
 // off, so the host build (used by the accuracy test) evaluates the same
@@ -59,25 +59,25 @@ TWMC_HD inline double fma3(double a, double b, double c) {
 #endif
 }
 
-// The series coefficients.  A kernel may hand the functions a copy held in
-// vector registers for its whole run (pin), instead of having every use form
-// its constant anew.
+// The series coefficients, highest degree first: the top one, then the six
+// others (near the Taylor coefficients 1/15!, -1/14!, 2/15, ..., which the fit
+// adjusts).  A kernel may hand the functions a copy held in vector registers
+// for its whole run (pin), instead of having every use form its constant anew.
 struct Series {
-  double s_top;  // -1/17!
-  double ks[7];  // 1/15!, -1/13!, ..., 1/3!
-  double c_top;  // 1/16!
-  double kc[7];  // -1/14!, 1/12!, ..., -1/2!
-  double l_top;  // 2/21
-  double kl[9];  // 2/19, 2/17, ..., 2/3
+  double s_top, ks[6];  // ps: ~1/15!, ~-1/13!, ..., 1/3!
+  double c_top, kc[6];  // pc: ~-1/14!, ~1/12!, ..., -1/2!
+  double l_top, kl[6];  // pl: ~2/15, ~2/13, ..., 2/3
 };
 constexpr Series kSeries = {
-    -1.0 / 355687428096000.0,
-    {1.0 / 1307674368000.0, -1.0 / 6227020800.0, 1.0 / 39916800.0, -1.0 / 362880.0, 1.0 / 5040.0, -1.0 / 120.0,
-     1.0 / 6.0},
-    1.0 / 20922789888000.0,
-    {-1.0 / 87178291200.0, 1.0 / 479001600.0, -1.0 / 3628800.0, 1.0 / 40320.0, -1.0 / 720.0, 1.0 / 24.0, -0.5},
-    2.0 / 21.0,
-    {2.0 / 19.0, 2.0 / 17.0, 2.0 / 15.0, 2.0 / 13.0, 2.0 / 11.0, 2.0 / 9.0, 2.0 / 7.0, 2.0 / 5.0, 2.0 / 3.0}};
+    7.586697117706918e-13,
+    {-1.6058531618986147e-10, 2.5052106232447578e-08, -2.7557319219339167e-06, 0.00019841269841265065,
+     -0.008333333333333331, 0.16666666666666666},
+    -1.1367998654022494e-11,
+    {2.0875886738047052e-09, -2.7557315566341895e-07, 2.480158729369346e-05, -0.0013888888888880775,
+     0.04166666666666664, -0.5},
+    0.14616449685043406,
+    {0.15331721600556042, 0.18182889125261723, 0.2222221113479508, 0.28571428625975487, 0.39999999999899505,
+     0.666666666666667}};
 
 // On the GPU: makes the coefficients of k values the compiler cannot see
 // through, so it keeps them in vector registers rather than forming each
@@ -85,8 +85,7 @@ constexpr Series kSeries = {
 TWMC_HD inline void pin(Series& k) {
 #if defined(__HIP_DEVICE_COMPILE__)
   asm volatile("" : "+v"(k.s_top), "+v"(k.c_top), "+v"(k.l_top));
-  for (int j = 0; j < 7; ++j) asm volatile("" : "+v"(k.ks[j]), "+v"(k.kc[j]));
-  for (int j = 0; j < 9; ++j) asm volatile("" : "+v"(k.kl[j]));
+  for (int j = 0; j < 6; ++j) asm volatile("" : "+v"(k.ks[j]), "+v"(k.kc[j]), "+v"(k.kl[j]));
 #else
   (void)k;
 #endif
@@ -120,9 +119,9 @@ TWMC_HD inline void sincos_turn32(const uint32_t* b, double* s, double* c, const
     ps[i] = k.s_top;
     pc[i] = k.c_top;
   }
-  // sin x = x - x z (1/3! - z/5! + ...), cos x = 1 + z (-1/2! + z/4! - ...), z = x^2
+  // sin x = x - x z ps(z), cos x = 1 + z pc(z), z = x^2
   TWMC_UNROLL
-  for (int j = 0; j < 7; ++j)
+  for (int j = 0; j < 6; ++j)
     TWMC_UNROLL
     for (int i = 0; i < N; ++i) {
       ps[i] = fma3(ps[i], z[i], k.ks[j]);
@@ -162,9 +161,9 @@ TWMC_HD inline void log_u32(const uint32_t* a, double* out, const Series& k = kS
     w[i] = f[i] * f[i];
     p[i] = k.l_top;
   }
-  // ln m = 2 f + f w (2/3 + 2 w/5 + ... + 2 w^9/21)
+  // ln m = 2 f + f w pl(w), w = f^2
   TWMC_UNROLL
-  for (int j = 0; j < 9; ++j)
+  for (int j = 0; j < 6; ++j)
     TWMC_UNROLL
     for (int i = 0; i < N; ++i) p[i] = fma3(p[i], w[i], k.kl[j]);
   TWMC_UNROLL
