@@ -240,6 +240,16 @@ void plan_segments(Ctx* c) {
   }
 }
 
+// rsl, U and F the same at every site and in every trajectory, F static: the
+// tables the init kernel forms are then uniform
+bool site_uniform(const Ctx* c) {
+  if (c->noisy_params || c->ft_n > 0) return false;
+  for (int k = 0; k < 3; ++k)
+    for (int i = 1; i < c->n; ++i)
+      if (c->base[k][2 * i] != c->base[k][0] || c->base[k][2 * i + 1] != c->base[k][1]) return false;
+  return true;
+}
+
 int choose_kernel(Ctx* c) {
   const twmc_params& p = c->p;
   // the register-resident kernel takes site-uniform, trajectory-shared parameters
@@ -477,7 +487,7 @@ int step_geometry(Ctx* c, size_t B, Geometry& geo) {
     if (c->p.hopping == TWMC_HOP_FFT) {
       int per_cu_nn = 0;
       geo.lds_nn = generic_nn_lds_bytes(c->n, c->precision);
-      HIPCHK(c, generic_nn_occupancy(c->precision, c->n, geo.lds_nn, &per_cu_nn));
+      HIPCHK(c, generic_nn_occupancy(c->precision, c->n, geo.lds_nn, site_uniform(c), &per_cu_nn));
       if (per_cu_nn < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");
       geo.grid_nn = (int)std::min<size_t>(B, (size_t)per_cu_nn * cus);
     }
@@ -521,6 +531,7 @@ StepArgs shared_step_args(const Ctx* c) {
   a.ft_times = static_cast<const double*>(c->d_ft_times.p);
   a.ft_vals = c->d_ft_vals.p;
   a.ft_slopes = c->d_ft_slopes.p;
+  a.site_uniform = site_uniform(c) ? 1 : 0;
   a.u_real = 1;
   for (int i = 0; i < c->n && a.u_real; ++i) a.u_real = c->base[1][2 * i + 1] == 0.0;
   if (c->noise[1] != TWMC_NOISE_NONE) a.u_real = 0;

@@ -59,7 +59,9 @@ __device__ __forceinline__ double warp_sum(double v) {
 // NN: a step launch with the hopping in its nearest-neighbour form and no
 // frame capture (the capture runs as its own launch): the kernel then needs
 // one LDS copy of the state and none of the FFT's registers.
-template <typename T, int PPT, bool NN = false>
+// UNIF (with NN): the site tables are uniform (StepArgs::site_uniform): their
+// first entries are read once, into registers, instead of per site and step.
+template <typename T, int PPT, bool NN = false, bool UNIF = false>
 __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = g.n;
@@ -101,6 +103,12 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
     const cpx<T>* rsl = reinterpret_cast<const cpx<T>*>(g.rsl) + poff;
     const cpx<T>* Up = reinterpret_cast<const cpx<T>*>(g.U) + poff;
     const cpx<T>* Fp = reinterpret_cast<const cpx<T>*>(g.F) + poff;
+    cpx<T> r0 = mk(T(0), T(0)), u0 = r0, f0 = r0;
+    if constexpr (UNIF) {
+      r0 = rsl[0];
+      u0 = Up[0];
+      f0 = Fp[0];
+    }
 
     cpx<T> beta[2 * PPT];
 #pragma unroll
@@ -280,7 +288,16 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
                   hh = mk(hh.re / T(n), hh.im / T(n));
                 }
               }
-              cpx<T> r = rsl[i], u = Up[i], f = fv[i];
+              cpx<T> r, u, f;
+              if constexpr (UNIF) {
+                r = r0;
+                u = u0;
+                f = f0;
+              } else {
+                r = rsl[i];
+                u = Up[i];
+                f = fv[i];
+              }
               if (fs) f = mk(f.re + fdt * fs[i].re, f.im + fdt * fs[i].im);
               T q = z.re * z.re + z.im * z.im - T(1);
               cpx<T> c = mk(r.re + (-u.im) * q, r.im + u.re * q);
@@ -503,12 +520,17 @@ hipError_t launch_generic_p(const StepArgs& a, int grid, int block, size_t lds, 
   }
 }
 
-template <typename T>
-const void* generic_nn_ptr(int ppt) {
-  return ppt == 1   ? (const void*)generic_step_kernel<T, 1, true>
-         : ppt == 2 ? (const void*)generic_step_kernel<T, 2, true>
-         : ppt == 8 ? (const void*)generic_step_kernel<T, 8, true>
-                    : (const void*)generic_step_kernel<T, 32, true>;
+template <typename T, bool UNIF>
+const void* generic_nn_ptr_u(int ppt) {
+  return ppt == 1   ? (const void*)generic_step_kernel<T, 1, true, UNIF>
+         : ppt == 2 ? (const void*)generic_step_kernel<T, 2, true, UNIF>
+         : ppt == 8 ? (const void*)generic_step_kernel<T, 8, true, UNIF>
+                    : (const void*)generic_step_kernel<T, 32, true, UNIF>;
+}
+
+const void* generic_nn_ptr(int precision, int ppt, bool unif) {
+  if (precision == 64) return unif ? generic_nn_ptr_u<double, true>(ppt) : generic_nn_ptr_u<double, false>(ppt);
+  return unif ? generic_nn_ptr_u<float, true>(ppt) : generic_nn_ptr_u<float, false>(ppt);
 }
 
 }  // namespace
@@ -516,9 +538,9 @@ const void* generic_nn_ptr(int ppt) {
 // LDS of a nearest-neighbour step launch: the per-wave sums and one copy of the state
 size_t generic_nn_lds_bytes(int n, int precision) { return 64 + (size_t)n * (precision == 64 ? 16 : 8); }
 
-hipError_t generic_nn_occupancy(int precision, int n, size_t lds, int* blocks_per_cu) {
+hipError_t generic_nn_occupancy(int precision, int n, size_t lds, bool site_uniform, int* blocks_per_cu) {
   const int ppt = generic_pairs_per_thread(n, kBlock);
-  const void* f = precision == 64 ? generic_nn_ptr<double>(ppt) : generic_nn_ptr<float>(ppt);
+  const void* f = generic_nn_ptr(precision, ppt, site_uniform);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -528,7 +550,7 @@ hipError_t generic_nn_occupancy(int precision, int n, size_t lds, int* blocks_pe
 
 hipError_t launch_generic_nn(const StepArgs& a, int precision, int grid, size_t lds, hipStream_t s) {
   const int ppt = generic_pairs_per_thread(a.n, kBlock);
-  const void* f = precision == 64 ? generic_nn_ptr<double>(ppt) : generic_nn_ptr<float>(ppt);
+  const void* f = generic_nn_ptr(precision, ppt, a.site_uniform != 0);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;

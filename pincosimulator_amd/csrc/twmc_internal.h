@@ -84,6 +84,7 @@ struct StepArgs {
   const void* a_lo;
   int np;                    // n padded to a multiple of 32
   int u_real;                // 1: Im U = 0 at every site, in every trajectory
+  int site_uniform;          // 1: rsl, U and F are the same at every site and in every trajectory, F static
   const float* band_coef;    // banded kernel: [64 S][2W + 1] dt C_(j+d) j
   int band_w;                // banded kernel: half-width W
   int band_lieb;             // banded kernel: the coupling has the Lieb chain's pattern (S = 6, W = 2)
@@ -177,7 +178,7 @@ int generic_pairs_per_thread(int n, int block);
 // the LDS kernel as a step launch with the hopping in its nearest-neighbour
 // form and no capture (FFT hopping): one LDS copy of the state
 size_t generic_nn_lds_bytes(int n, int precision);
-hipError_t generic_nn_occupancy(int precision, int n, size_t lds, int* blocks_per_cu);
+hipError_t generic_nn_occupancy(int precision, int n, size_t lds, bool site_uniform, int* blocks_per_cu);
 hipError_t launch_generic_nn(const StepArgs& a, int precision, int grid, size_t lds, hipStream_t s);
 hipError_t generic_occupancy(int precision, int n, int block, size_t lds, int* blocks_per_cu);
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
