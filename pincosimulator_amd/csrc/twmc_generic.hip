@@ -103,11 +103,13 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
     const cpx<T>* rsl = reinterpret_cast<const cpx<T>*>(g.rsl) + poff;
     const cpx<T>* Up = reinterpret_cast<const cpx<T>*>(g.U) + poff;
     const cpx<T>* Fp = reinterpret_cast<const cpx<T>*>(g.F) + poff;
-    cpx<T> r0 = mk(T(0), T(0)), u0 = r0, f0 = r0;
+    // UNIF: the site constants pre-scaled by dt, dt rsl, dt U and dt i F
+    cpx<T> rd = mk(T(0), T(0)), ud = rd, fd = rd;
     if constexpr (UNIF) {
-      r0 = rsl[0];
-      u0 = Up[0];
-      f0 = Fp[0];
+      const cpx<T> r0 = rsl[0], u0 = Up[0], f0 = Fp[0];
+      rd = mk(dt * r0.re, dt * r0.im);
+      ud = mk(dt * u0.re, dt * u0.im);
+      fd = mk(-dt * f0.im, dt * f0.re);
     }
 
     cpx<T> beta[2 * PPT];
@@ -288,23 +290,31 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
                   hh = mk(hh.re / T(n), hh.im / T(n));
                 }
               }
-              cpx<T> r, u, f;
               if constexpr (UNIF) {
-                r = r0;
-                u = u0;
-                f = f0;
+                // the same update with dt folded into the constants: the
+                // increment dt (c z + h + i F) + dt4 xi as one FMA chain per
+                // component, added to beta once
+                const T q = fma(z.re, z.re, fma(z.im, z.im, T(-1)));
+                const T cr = fma(-ud.im, q, rd.re), ci = fma(ud.re, q, rd.im);
+                T dr = fma(dt4, xr[h], fd.re);
+                dr = fma(dt, hh.re, dr);
+                dr = fma(cr, z.re, dr);
+                dr = fma(-ci, z.im, dr);
+                T di = fma(dt4, xi[h], fd.im);
+                di = fma(dt, hh.im, di);
+                di = fma(cr, z.im, di);
+                di = fma(ci, z.re, di);
+                z = mk(z.re + dr, z.im + di);
               } else {
-                r = rsl[i];
-                u = Up[i];
-                f = fv[i];
+                cpx<T> r = rsl[i], u = Up[i], f = fv[i];
+                if (fs) f = mk(f.re + fdt * fs[i].re, f.im + fdt * fs[i].im);
+                T q = z.re * z.re + z.im * z.im - T(1);
+                cpx<T> c = mk(r.re + (-u.im) * q, r.im + u.re * q);
+                cpx<T> cb = cmul(c, z);
+                cpx<T> sdr = mk(cb.re + hh.re + (-f.im), cb.im + hh.im + f.re);
+                cpx<T> a = mk(sdr.re * dt, sdr.im * dt);
+                z = mk(z.re + a.re + dt4 * xr[h], z.im + a.im + dt4 * xi[h]);
               }
-              if (fs) f = mk(f.re + fdt * fs[i].re, f.im + fdt * fs[i].im);
-              T q = z.re * z.re + z.im * z.im - T(1);
-              cpx<T> c = mk(r.re + (-u.im) * q, r.im + u.re * q);
-              cpx<T> cb = cmul(c, z);
-              cpx<T> sdr = mk(cb.re + hh.re + (-f.im), cb.im + hh.im + f.re);
-              cpx<T> a = mk(sdr.re * dt, sdr.im * dt);
-              z = mk(z.re + a.re + dt4 * xr[h], z.im + a.im + dt4 * xi[h]);
               beta[2 * j + h] = z;
             }
           }
