@@ -487,7 +487,7 @@ int step_geometry(Ctx* c, size_t B, Geometry& geo) {
     if (c->p.hopping == TWMC_HOP_FFT) {
       int per_cu_nn = 0;
       geo.lds_nn = generic_nn_lds_bytes(c->n, c->precision);
-      HIPCHK(c, generic_nn_occupancy(c->precision, c->n, geo.lds_nn, site_uniform(c), &per_cu_nn));
+      HIPCHK(c, generic_nn_occupancy(c->precision, c->p.nx, c->p.ny, geo.lds_nn, site_uniform(c), &per_cu_nn));
       if (per_cu_nn < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");
       geo.grid_nn = (int)std::min<size_t>(B, (size_t)per_cu_nn * cus);
     }

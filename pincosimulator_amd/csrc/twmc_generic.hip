@@ -61,7 +61,9 @@ __device__ __forceinline__ double warp_sum(double v) {
 // one LDS copy of the state and none of the FFT's registers.
 // UNIF (with NN): the site tables are uniform (StepArgs::site_uniform): their
 // first entries are read once, into registers, instead of per site and step.
-template <typename T, int PPT, bool NN = false, bool UNIF = false>
+// P2 (with NN): ny a power of two >= 2 and nx = 1 or a power of two >= 2, so
+// the neighbour indices are masks instead of a division and wrap tests.
+template <typename T, int PPT, bool NN = false, bool UNIF = false, bool P2 = false>
 __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = g.n;
@@ -270,9 +272,23 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
                 if constexpr (kStencil) {
                   // h = -i J S, S the sum over the periodic nearest neighbours
                   // (a dimension of length 1 has none): IDFT(K DFT beta)/N exactly
+                  T sr = T(0), si = T(0);
+                  if constexpr (P2) {
+                    const int ym = g.ny - 1, y = i & ym, rb = i - y;
+                    const cpx<T> a = bufA[rb + ((y + 1) & ym)];
+                    const cpx<T> c = bufA[rb + ((y - 1) & ym)];
+                    sr = a.re + c.re;
+                    si = a.im + c.im;
+                    if (g.nx > 1) {
+                      const int nm = n - 1;
+                      const cpx<T> d = bufA[(i + g.ny) & nm];
+                      const cpx<T> e = bufA[(i - g.ny) & nm];
+                      sr += d.re + e.re;
+                      si += d.im + e.im;
+                    }
+                  } else {
                   const int x = g.ny > 1 ? (int)__umulhi((uint32_t)i, ymagic) : i;
                   const int y = i - x * g.ny;
-                  T sr = T(0), si = T(0);
                   if (g.ny > 1) {
                     const cpx<T> a = bufA[x * g.ny + (y + 1 == g.ny ? 0 : y + 1)];
                     const cpx<T> c = bufA[x * g.ny + (y == 0 ? g.ny - 1 : y - 1)];
@@ -284,6 +300,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
                     const cpx<T> c = bufA[(x == 0 ? g.nx - 1 : x - 1) * g.ny + y];
                     sr += a.re + c.re;
                     si += a.im + c.im;
+                  }
                   }
                   hh = mk(jval * si, -jval * sr);
                 } else {
@@ -530,17 +547,28 @@ hipError_t launch_generic_p(const StepArgs& a, int grid, int block, size_t lds, 
   }
 }
 
-template <typename T, bool UNIF>
+template <typename T, bool UNIF, bool P2>
 const void* generic_nn_ptr_u(int ppt) {
-  return ppt == 1   ? (const void*)generic_step_kernel<T, 1, true, UNIF>
-         : ppt == 2 ? (const void*)generic_step_kernel<T, 2, true, UNIF>
-         : ppt == 8 ? (const void*)generic_step_kernel<T, 8, true, UNIF>
-                    : (const void*)generic_step_kernel<T, 32, true, UNIF>;
+  return ppt == 1   ? (const void*)generic_step_kernel<T, 1, true, UNIF, P2>
+         : ppt == 2 ? (const void*)generic_step_kernel<T, 2, true, UNIF, P2>
+         : ppt == 8 ? (const void*)generic_step_kernel<T, 8, true, UNIF, P2>
+                    : (const void*)generic_step_kernel<T, 32, true, UNIF, P2>;
 }
 
-const void* generic_nn_ptr(int precision, int ppt, bool unif) {
-  if (precision == 64) return unif ? generic_nn_ptr_u<double, true>(ppt) : generic_nn_ptr_u<double, false>(ppt);
-  return unif ? generic_nn_ptr_u<float, true>(ppt) : generic_nn_ptr_u<float, false>(ppt);
+// power-of-two lattices (the mask form of the neighbour indices) are
+// instantiated for site-uniform parameters only
+bool pow2_lattice(int nx, int ny) {
+  auto p2 = [](int v) { return v >= 2 && (v & (v - 1)) == 0; };
+  return p2(ny) && (nx == 1 || p2(nx));
+}
+
+const void* generic_nn_ptr(int precision, int ppt, bool unif, bool p2) {
+  if (precision == 64) {
+    if (unif) return p2 ? generic_nn_ptr_u<double, true, true>(ppt) : generic_nn_ptr_u<double, true, false>(ppt);
+    return generic_nn_ptr_u<double, false, false>(ppt);
+  }
+  if (unif) return p2 ? generic_nn_ptr_u<float, true, true>(ppt) : generic_nn_ptr_u<float, true, false>(ppt);
+  return generic_nn_ptr_u<float, false, false>(ppt);
 }
 
 }  // namespace
@@ -548,9 +576,10 @@ const void* generic_nn_ptr(int precision, int ppt, bool unif) {
 // LDS of a nearest-neighbour step launch: the per-wave sums and one copy of the state
 size_t generic_nn_lds_bytes(int n, int precision) { return 64 + (size_t)n * (precision == 64 ? 16 : 8); }
 
-hipError_t generic_nn_occupancy(int precision, int n, size_t lds, bool site_uniform, int* blocks_per_cu) {
-  const int ppt = generic_pairs_per_thread(n, kBlock);
-  const void* f = generic_nn_ptr(precision, ppt, site_uniform);
+hipError_t generic_nn_occupancy(int precision, int nx, int ny, size_t lds, bool site_uniform,
+                                int* blocks_per_cu) {
+  const int ppt = generic_pairs_per_thread(nx * ny, kBlock);
+  const void* f = generic_nn_ptr(precision, ppt, site_uniform, pow2_lattice(nx, ny));
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -560,7 +589,7 @@ hipError_t generic_nn_occupancy(int precision, int n, size_t lds, bool site_unif
 
 hipError_t launch_generic_nn(const StepArgs& a, int precision, int grid, size_t lds, hipStream_t s) {
   const int ppt = generic_pairs_per_thread(a.n, kBlock);
-  const void* f = generic_nn_ptr(precision, ppt, a.site_uniform != 0);
+  const void* f = generic_nn_ptr(precision, ppt, a.site_uniform != 0, pow2_lattice(a.nx, a.ny));
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;

@@ -1,8 +1,9 @@
 """The LDS kernel on awkward shapes -- dimensions of length 1, 2, odd and
-prime lengths (the FFT's direct-sum passes for prime factors above 5) --
-against the oracle that applies the reference's FFT form.  FP32 (hopping as
-the nearest-neighbour form, the FFT only at captures): 2e-5 of max |beta|
-after 200 steps.  FP64 (the FFT form every step): 1e-9."""
+prime lengths (the FFT's direct-sum passes for prime factors above 5) -- and
+on power-of-two lattices (the neighbour indices as masks), against the oracle
+that applies the reference's FFT form.  The step launches use the hopping's
+nearest-neighbour form and the FFT only at captures.  FP32: 2e-5 of max
+|beta| after 200 steps; FP64: 1e-9."""
 import numpy as np
 import pytest
 
@@ -13,7 +14,7 @@ from pincosimulator_amd import Engine
 pytestmark = pytest.mark.gpu
 
 
-SHAPES = [(1, 2), (1, 3), (2, 2), (2, 5), (16, 15), (5, 7), (1, 60), (1, 127), (7, 11), (13, 14)]
+SHAPES = [(1, 2), (1, 3), (2, 2), (2, 5), (16, 15), (5, 7), (1, 60), (1, 127), (7, 11), (13, 14), (4, 32), (32, 1)]
 
 
 @pytest.mark.parametrize("nx,ny", SHAPES)
@@ -36,8 +37,11 @@ def test_fp32_nearest_neighbour_matches_fft_oracle(nx, ny):
     assert rel_err(res.sum_nk, nk_ref.reshape(res.sum_nk.shape)) < 2e-5
 
 
-@pytest.mark.parametrize("nx,ny", [(1, 7), (5, 7), (1, 127), (7, 11), (13, 14)])
-def test_fp64_prime_lengths_match_oracle(nx, ny):
+# power-of-two lattices (ny >= 2, nx = 1 or >= 2) take the step launch whose
+# neighbour indices are masks; (8, 6) and (32, 1) the division form
+@pytest.mark.parametrize("nx,ny", [(1, 7), (5, 7), (1, 127), (7, 11), (13, 14),
+                                   (1, 2), (1, 32), (2, 8), (4, 2), (16, 4), (8, 6), (32, 1)])
+def test_fp64_awkward_and_power_of_two_shapes_match_oracle(nx, ny):
     spec = dict(nx=nx, ny=ny, dt=0.005, gamma=1.0, J=1.3, omega=0.4, U=0.1, F=1.1, beta_init=0.2,
                 beta_sigma=0.5, frame_steps=50, seed=516, t_start=0.0, t_end=0.5)
     ids = np.arange(4) + 7
