@@ -6,15 +6,21 @@
 // reduce exactly in integer arithmetic and evaluate short polynomials:
 //   sin, cos: the quarter turn nearest to b is removed from b itself, leaving
 //     an angle |x| <= pi/4; sin x = x - x^3 ps(x^2), cos x = 1 + x^2 pc(x^2);
-//   ln: u1 = m 2^e with m in (sqrt(1/2), sqrt(2)] from the exponent bits,
+//   ln: u1 = m 2^e with m within 1e-6 of (sqrt(1/2), sqrt(2)) from the
+//     exponent and significand bits of a + 1/2,
 //     ln m = 2 atanh(f) = 2 f + f^3 pl(f^2), f = (m - 1)/(m + 1),
 //     |f| <= 0.1716, plus e ln 2 with ln 2 split so that e ln2_hi is exact;
 //   ps, pc, pl: degree-6 Chebyshev fits over those ranges (gen_f64_series.py),
-//     each adding less than 2e-17 to the relative error of its function.
+//     each adding less than 2e-17 to the relative error of its function;
+//   the quotient f and the radius sqrt(-2 ln u1) by Newton steps from the
+//     hardware reciprocal and reciprocal square root: their arguments
+//     (1.7 < m + 1 < 2.5, 2e-10 < -2 ln u1 < 46) need none of the library
+//     division's or square root's scaling for extreme exponents.
 // N samples are evaluated side by side, so each series coefficient is formed
 // once for all N.  Every multiply-add is an explicit fma and contraction is
 // off, so the host build (used by the accuracy test) evaluates the same
-// correctly rounded operations in the same order as the device build.  Pure
+// correctly rounded operations in the same order as the device build; only
+// the Newton seeds differ (a coarser one on the host, see rcp_seed).  Pure
 // C++: no HIP header is needed to compile it on the host.
 #ifndef TWMC_F64MATH_H
 #define TWMC_F64MATH_H
@@ -59,6 +65,66 @@ TWMC_HD inline double fma3(double a, double b, double c) {
 #endif
 }
 
+TWMC_HD inline uint64_t bits(double x) {
+  uint64_t u;
+  memcpy(&u, &x, sizeof u);
+  return u;
+}
+
+TWMC_HD inline double from_bits(uint64_t u) {
+  double x;
+  memcpy(&x, &u, sizeof x);
+  return x;
+}
+
+// Seeds for the Newton steps: the hardware's approximate 1/d and 1/sqrt(x) on
+// the GPU.  On the host the correctly rounded value cut to 21 significant bits,
+// a seed no better than the hardware's, so the accuracy test of the host build
+// covers the refinement from a coarse start.
+TWMC_HD inline double rcp_seed(double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcp(d);
+#else
+  return from_bits(bits(1.0 / d) & 0xffffffff00000000ull);
+#endif
+}
+
+TWMC_HD inline double rsq_seed(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsq(x);
+#else
+  return from_bits(bits(1.0 / sqrt(x)) & 0xffffffff00000000ull);
+#endif
+}
+
+// n / d for 1 < d < 4 (no scaling): two Newton steps on the reciprocal
+// (each squares its relative error), the quotient, and one correction from
+// the exact residual n - d q.
+TWMC_HD inline double div_small(double n, double d) {
+  TWMC_NO_CONTRACT
+  double r = rcp_seed(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  const double q = n * r;
+  return fma(fma(-d, q, n), r, q);
+}
+
+// sqrt(x) for normal x well inside the exponent range (no scaling): from
+// y ~ 1/sqrt(x), g ~ sqrt(x) and h ~ 1/(2 sqrt(x)) are refined together
+// (Goldschmidt: r = 1/2 - g h is the shared relative error, squared by one
+// step), then g once more by the exact residual x - g^2.
+TWMC_HD inline double sqrt_normal(double x) {
+  TWMC_NO_CONTRACT
+  const double y = rsq_seed(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  return fma(fma(-g, g, x), h, g);
+}
+
 // The series coefficients, highest degree first: the top one, then the six
 // others (near the Taylor coefficients 1/15!, -1/14!, 2/15, ..., which the fit
 // adjusts).  A kernel may hand the functions a copy held in vector registers
@@ -89,18 +155,6 @@ TWMC_HD inline void pin(Series& k) {
 #else
   (void)k;
 #endif
-}
-
-TWMC_HD inline uint64_t bits(double x) {
-  uint64_t u;
-  memcpy(&u, &x, sizeof u);
-  return u;
-}
-
-TWMC_HD inline double from_bits(uint64_t u) {
-  double x;
-  memcpy(&x, &u, sizeof x);
-  return x;
 }
 
 // sin and cos of 2 pi b[i] 2^-32, i < N.
@@ -147,17 +201,17 @@ TWMC_HD inline void log_u32(const uint32_t* a, double* out, const Series& k = kS
   int e[N];
   TWMC_UNROLL
   for (int i = 0; i < N; ++i) {
-    const double u = (double(a[i]) + 0.5) * (1.0 / 4294967296.0);  // exact, normal
-    const uint64_t ub = bits(u);
-    // u = m 2^e with m in [1, 2), then m in (sqrt(1/2), sqrt(2)]
-    e[i] = int((ub >> 52) & 0x7ff) - 1023;
-    uint64_t mb = (ub & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
-    if (mb > 0x3ff6a09e667f3bcdull) {  // m > sqrt(2) (the double nearest it)
-      mb -= 0x0010000000000000ull;     // m / 2, exactly
-      ++e[i];
-    }
-    const double m = from_bits(mb);
-    f[i] = (m - 1.0) / (m + 1.0);  // m - 1 exact
+    // a + 1/2 = m 2^(e + 32) exactly, m from the significand: in [1, 2), or
+    // halved when its top 20 fraction bits exceed sqrt(2)'s (0x6a09e), so m in
+    // [0.7071075, 1.4142151) -- the test on the high word alone widens the
+    // interval around sqrt(2) by 1e-6, inside the series' fitted range of f
+    const uint64_t vb = bits(double(a[i]) + 0.5);
+    const uint32_t hi = uint32_t(vb >> 32), t = hi & 0xfffffu;
+    const bool half = t > 0x6a09eu;
+    e[i] = int(hi >> 20) - (1023 + 32) + (half ? 1 : 0);
+    const uint32_t mh = t | (half ? 0x3fe00000u : 0x3ff00000u);
+    const double m = from_bits((uint64_t(mh) << 32) | (vb & 0xffffffffull));
+    f[i] = div_small(m - 1.0, m + 1.0);  // m - 1 exact
     w[i] = f[i] * f[i];
     p[i] = k.l_top;
   }
@@ -185,7 +239,7 @@ TWMC_HD inline void box_muller(const uint32_t* a, const uint32_t* b, double* re,
   sincos_turn32<N>(b, s, c, k);
   TWMC_UNROLL
   for (int i = 0; i < N; ++i) {
-    const double r = sqrt(-2.0 * l[i]);
+    const double r = sqrt_normal(-2.0 * l[i]);
     re[i] = r * c[i];
     im[i] = r * s[i];
   }

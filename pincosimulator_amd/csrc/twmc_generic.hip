@@ -65,6 +65,8 @@ __device__ __forceinline__ double warp_sum(double v) {
 // the neighbour indices are masks instead of a division and wrap tests.
 template <typename T, int PPT, bool NN = false, bool UNIF = false, bool P2 = false>
 __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
+  static_assert(!UNIF || NN, "site-uniform launches are nearest-neighbour launches");
+  static_assert(!P2 || UNIF, "the mask form is instantiated for site-uniform launches");
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = g.n;
   double* red = reinterpret_cast<double*>(smem);  // 8 doubles: per-wave partial sums
@@ -97,6 +99,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
   const uint32_t ymagic = g.ny > 1 ? (uint32_t)((0x100000000ull + (uint64_t)g.ny - 1) / (uint64_t)g.ny) : 0u;
   const T jval = T(-0.5 * g.kscale);  // J
   const T dt = T(g.dt), dt4 = T(g.dt4), kscale = T(g.kscale);
+  const T djv = T(-0.5 * g.kscale * g.dt);  // dt J
 
   for (int b = blockIdx.x; b < g.n_traj; b += gridDim.x) {
     const uint64_t traj = g.traj_ids[b];
@@ -262,27 +265,40 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
           u32x4 x = philox4x32_10(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
           T xr[2], xi[2];
           box_muller_pair<T>(x, xr, xi, bmk);
+          // P2: the pair's sites 2p, 2p + 1 share a row (ny even) and are each
+          // other's y neighbours; their other y neighbours are rb + (y0 - 1) and
+          // rb + (y0 + 2) (mod ny), their x neighbours adjacent pairs of sites.
+          // Both sites' new values are formed before either is stored, so beta
+          // holds the pair's state before this step throughout (P2).
+          cpx<T> znew[2];
+          int nbl = 0, nbr = 0, nbu = 0, nbd = 0;
+          if constexpr (P2) {
+            const int i0 = 2 * p + opq, ym = g.ny - 1, y0 = i0 & ym, rb = i0 - y0, nm = n - 1;
+            nbl = rb + ((y0 - 1) & ym);
+            nbr = rb + ((y0 + 2) & ym);
+            nbu = (i0 + g.ny) & nm;
+            nbd = (i0 - g.ny) & nm;
+          }
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             int i = 2 * p + h + opq;
-            if (i < n) {
+            if (P2 || i < n) {  // P2: n even
               cpx<T> z = beta[2 * j + h];
               cpx<T> hh = cur[i];
+              T sr = T(0), si = T(0);
               if (NN || g.hopping == 0) {
                 if constexpr (kStencil) {
                   // h = -i J S, S the sum over the periodic nearest neighbours
                   // (a dimension of length 1 has none): IDFT(K DFT beta)/N exactly
-                  T sr = T(0), si = T(0);
                   if constexpr (P2) {
-                    const int ym = g.ny - 1, y = i & ym, rb = i - y;
-                    const cpx<T> a = bufA[rb + ((y + 1) & ym)];
-                    const cpx<T> c = bufA[rb + ((y - 1) & ym)];
+                    // y + 1 and y - 1 neighbours, then x + 1 and x - 1
+                    const cpx<T> a = h == 0 ? beta[2 * j + 1] : bufA[nbr];
+                    const cpx<T> c = h == 0 ? bufA[nbl] : beta[2 * j];
                     sr = a.re + c.re;
                     si = a.im + c.im;
                     if (g.nx > 1) {
-                      const int nm = n - 1;
-                      const cpx<T> d = bufA[(i + g.ny) & nm];
-                      const cpx<T> e = bufA[(i - g.ny) & nm];
+                      const cpx<T> d = bufA[nbu + h];
+                      const cpx<T> e = bufA[nbd + h];
                       sr += d.re + e.re;
                       si += d.im + e.im;
                     }
@@ -310,15 +326,15 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
               if constexpr (UNIF) {
                 // the same update with dt folded into the constants: the
                 // increment dt (c z + h + i F) + dt4 xi as one FMA chain per
-                // component, added to beta once
+                // component, added to beta once; dt h = dt J (S_im, -S_re)
                 const T q = fma(z.re, z.re, fma(z.im, z.im, T(-1)));
                 const T cr = fma(-ud.im, q, rd.re), ci = fma(ud.re, q, rd.im);
                 T dr = fma(dt4, xr[h], fd.re);
-                dr = fma(dt, hh.re, dr);
+                dr = fma(djv, si, dr);
                 dr = fma(cr, z.re, dr);
                 dr = fma(-ci, z.im, dr);
                 T di = fma(dt4, xi[h], fd.im);
-                di = fma(dt, hh.im, di);
+                di = fma(-djv, sr, di);
                 di = fma(cr, z.im, di);
                 di = fma(ci, z.re, di);
                 z = mk(z.re + dr, z.im + di);
@@ -332,8 +348,15 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
                 cpx<T> a = mk(sdr.re * dt, sdr.im * dt);
                 z = mk(z.re + a.re + dt4 * xr[h], z.im + a.im + dt4 * xi[h]);
               }
-              beta[2 * j + h] = z;
+              if constexpr (P2)
+                znew[h] = z;
+              else
+                beta[2 * j + h] = z;
             }
+          }
+          if constexpr (P2) {
+            beta[2 * j] = znew[0];
+            beta[2 * j + 1] = znew[1];
           }
         }
       }

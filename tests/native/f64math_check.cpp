@@ -6,7 +6,8 @@
 // Inputs: every 32-bit a / b of the form k * 4093 + (k * 2654435761 mod 4093)
 // for k below 2^20 (1M values spread over the whole range), plus the edges
 // 0, 1, 2^32 - 1, every power of two, and the quarter- and eighth-turn
-// boundaries of b with their neighbours.
+// boundaries of b with their neighbours, and a + 1/2 at and just above
+// sqrt(2) 2^p, where ln's reduction switches.
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -36,6 +37,11 @@ int main() {
   }
   for (uint32_t j = 0; j < 8; ++j)  // eighth turns: the reduction's boundaries
     for (int d = -2; d <= 2; ++d) xs.push_back(j * 0x20000000u + (uint32_t)d);
+  for (int p = 1; p < 32; ++p) {  // ln's reduction boundary: a + 1/2 near sqrt(2) 2^p
+    const uint32_t c = (uint32_t)(1.4142135623730951 * (double)(1u << p));
+    for (int d = -3; d <= 3; ++d) xs.push_back(c + (uint32_t)d);
+    for (uint32_t d = 0; d < 4000u; d += 37u) xs.push_back(c + d);
+  }
 
   const long double two_pi = 6.283185307179586476925286766559L;
   double e_log = 0, e_sin = 0, e_cos = 0, e_bm = 0;
