@@ -6,21 +6,21 @@
 // reduce exactly in integer arithmetic and evaluate short polynomials:
 //   sin, cos: the quarter turn nearest to b is removed from b itself, leaving
 //     an angle |x| <= pi/4; sin x = x - x^3 ps(x^2), cos x = 1 + x^2 pc(x^2);
-//   ln: u1 = m 2^e with m within 1e-6 of (sqrt(1/2), sqrt(2)) from the
-//     exponent and significand bits of a + 1/2,
-//     ln m = 2 atanh(f) = 2 f + f^3 pl(f^2), f = (m - 1)/(m + 1),
-//     |f| <= 0.1716, plus e ln 2 with ln 2 split so that e ln2_hi is exact;
-//   ps, pc, pl: degree-6 Chebyshev fits over those ranges (gen_f64_series.py),
+//   ln: u1 = m 2^e from the exponent and significand bits of a + 1/2, m in
+//     [1, 2); a 128-row table (twmc_f64tab.h) indexed by m's top 7 fraction
+//     bits gives c_j and L_j with ln m = L_j + ln(1 + r), r = m c_j - 1
+//     exact and |r| <= 2^-7, ln(1 + r) = r + r^2 pq(r), plus e ln 2 with ln 2
+//     split so that e ln2_hi is exact;
+//   ps, pc, pq: degree-6 Chebyshev fits over those ranges (gen_f64_series.py),
 //     each adding less than 2e-17 to the relative error of its function;
-//   the quotient f and the radius sqrt(-2 ln u1) by Newton steps from the
-//     hardware reciprocal and reciprocal square root: their arguments
-//     (1.7 < m + 1 < 2.5, 2e-10 < -2 ln u1 < 46) need none of the library
-//     division's or square root's scaling for extreme exponents.
+//   the radius sqrt(-2 ln u1) by Newton steps from the hardware reciprocal
+//     square root: its argument (2e-10 < -2 ln u1 < 46) needs none of the
+//     library square root's scaling for extreme exponents.
 // N samples are evaluated side by side, so each series coefficient is formed
 // once for all N.  Every multiply-add is an explicit fma and contraction is
 // off, so the host build (used by the accuracy test) evaluates the same
 // correctly rounded operations in the same order as the device build; only
-// the Newton seeds differ (a coarser one on the host, see rcp_seed).  Pure
+// the Newton seed differs (a coarser one on the host, see rsq_seed).  Pure
 // C++: no HIP header is needed to compile it on the host.
 #ifndef TWMC_F64MATH_H
 #define TWMC_F64MATH_H
@@ -28,6 +28,8 @@
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
+
+#include "twmc_f64tab.h"
 
 #if defined(__HIPCC__)
 #define TWMC_HD __host__ __device__
@@ -77,38 +79,16 @@ TWMC_HD inline double from_bits(uint64_t u) {
   return x;
 }
 
-// Seeds for the Newton steps: the hardware's approximate 1/d and 1/sqrt(x) on
-// the GPU.  On the host the correctly rounded value cut to 21 significant bits,
-// a seed no better than the hardware's, so the accuracy test of the host build
-// covers the refinement from a coarse start.
-TWMC_HD inline double rcp_seed(double d) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __builtin_amdgcn_rcp(d);
-#else
-  return from_bits(bits(1.0 / d) & 0xffffffff00000000ull);
-#endif
-}
-
+// Seed for the square root's Newton steps: the hardware's approximate
+// 1/sqrt(x) on the GPU.  On the host the correctly rounded value cut to 21
+// significant bits, a seed no better than the hardware's, so the accuracy test
+// of the host build covers the refinement from a coarse start.
 TWMC_HD inline double rsq_seed(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __builtin_amdgcn_rsq(x);
 #else
   return from_bits(bits(1.0 / sqrt(x)) & 0xffffffff00000000ull);
 #endif
-}
-
-// n / d for 1 < d < 4 (no scaling): two Newton steps on the reciprocal
-// (each squares its relative error), the quotient, and one correction from
-// the exact residual n - d q.
-TWMC_HD inline double div_small(double n, double d) {
-  TWMC_NO_CONTRACT
-  double r = rcp_seed(d);
-  double e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
-  const double q = n * r;
-  return fma(fma(-d, q, n), r, q);
 }
 
 // sqrt(x) for normal x well inside the exponent range (no scaling): from
@@ -126,13 +106,13 @@ TWMC_HD inline double sqrt_normal(double x) {
 }
 
 // The series coefficients, highest degree first: the top one, then the six
-// others (near the Taylor coefficients 1/15!, -1/14!, 2/15, ..., which the fit
+// others (near the Taylor coefficients 1/15!, -1/14!, -1/8, ..., which the fit
 // adjusts).  A kernel may hand the functions a copy held in vector registers
 // for its whole run (pin), instead of having every use form its constant anew.
 struct Series {
   double s_top, ks[6];  // ps: ~1/15!, ~-1/13!, ..., 1/3!
   double c_top, kc[6];  // pc: ~-1/14!, ~1/12!, ..., -1/2!
-  double l_top, kl[6];  // pl: ~2/15, ~2/13, ..., 2/3
+  double l_top, kl[6];  // pq: ~-1/8, ~1/7, ..., -1/2
 };
 constexpr Series kSeries = {
     7.586697117706918e-13,
@@ -141,9 +121,9 @@ constexpr Series kSeries = {
     -1.1367998654022494e-11,
     {2.0875886738047052e-09, -2.7557315566341895e-07, 2.480158729369346e-05, -0.0013888888888880775,
      0.04166666666666664, -0.5},
-    0.14616449685043406,
-    {0.15331721600556042, 0.18182889125261723, 0.2222221113479508, 0.28571428625975487, 0.39999999999899505,
-     0.666666666666667}};
+    -0.1250106818314722,
+    {0.14286901154505832, -0.1666666663406768, 0.1999999996377896, -0.2500000000000025, 0.3333333333333361,
+     -0.5}};
 
 // On the GPU: makes the coefficients of k values the compiler cannot see
 // through, so it keeps them in vector registers rather than forming each
@@ -193,38 +173,50 @@ TWMC_HD inline void sincos_turn32(const uint32_t* b, double* s, double* c, const
   }
 }
 
-// ln((a[i] + 1/2) 2^-32), in (-22.9, 0), i < N.
+// One row of the ln table: r = m c - 1 and ln m = L + ln(1 + r) (with the
+// exponent adjusted by one for m > sqrt 2, which L absorbs as -ln 2).
+struct alignas(16) LogRow {
+  double c, L;
+};
+
+// The table's 128 rows as 256 doubles (twmc_f64tab.h).  On the host the
+// functions read it from here; device code hands them a copy in device memory.
+constexpr double kLogTab[256] = TWMC_F64_LOG_TABLE;
+
+// ln((a[i] + 1/2) 2^-32), in (-22.9, 0), i < N; tab: the 256 doubles of
+// TWMC_F64_LOG_TABLE.
 template <int N>
-TWMC_HD inline void log_u32(const uint32_t* a, double* out, const Series& k = kSeries) {
+TWMC_HD inline void log_u32(const uint32_t* a, double* out, const Series& k = kSeries,
+                            const double* tab = kLogTab) {
   TWMC_NO_CONTRACT
-  double f[N], w[N], p[N];
+  const LogRow* rows = reinterpret_cast<const LogRow*>(tab);
+  double r[N], lt[N], p[N];
   int e[N];
   TWMC_UNROLL
   for (int i = 0; i < N; ++i) {
-    // a + 1/2 = m 2^(e + 32) exactly, m from the significand: in [1, 2), or
-    // halved when its top 20 fraction bits exceed sqrt(2)'s (0x6a09e), so m in
-    // [0.7071075, 1.4142151) -- the test on the high word alone widens the
-    // interval around sqrt(2) by 1e-6, inside the series' fitted range of f
+    // a + 1/2 = m 2^(E - 1023) exactly, m in [1, 2) with at most 33 significant
+    // bits; row j from its top 7 fraction bits; u = m 2^(E - 1055)
     const uint64_t vb = bits(double(a[i]) + 0.5);
-    const uint32_t hi = uint32_t(vb >> 32), t = hi & 0xfffffu;
-    const bool half = t > 0x6a09eu;
-    e[i] = int(hi >> 20) - (1023 + 32) + (half ? 1 : 0);
-    const uint32_t mh = t | (half ? 0x3fe00000u : 0x3ff00000u);
-    const double m = from_bits((uint64_t(mh) << 32) | (vb & 0xffffffffull));
-    f[i] = div_small(m - 1.0, m + 1.0);  // m - 1 exact
-    w[i] = f[i] * f[i];
+    const uint32_t hi = uint32_t(vb >> 32);
+    const uint32_t j = (hi >> 13) & 127u;
+    // E - 1055, plus one for rows j >= 53: adding 75 to j carries into E
+    e[i] = int((hi + (75u << 13)) >> 20) - 1055;
+    const double m = from_bits((uint64_t((hi & 0xfffffu) | 0x3ff00000u) << 32) | (vb & 0xffffffffull));
+    const LogRow row = rows[j];
+    r[i] = fma(m, row.c, -1.0);  // exact: |r| <= 2^-7, m c has at most 52 bits
+    lt[i] = row.L;
     p[i] = k.l_top;
   }
-  // ln m = 2 f + f w pl(w), w = f^2
+  // ln(1 + r) = r + r^2 pq(r)
   TWMC_UNROLL
   for (int j = 0; j < 6; ++j)
     TWMC_UNROLL
-    for (int i = 0; i < N; ++i) p[i] = fma3(p[i], w[i], k.kl[j]);
+    for (int i = 0; i < N; ++i) p[i] = fma3(p[i], r[i], k.kl[j]);
   TWMC_UNROLL
   for (int i = 0; i < N; ++i) {
-    const double lnm = fma(f[i] * w[i], p[i], f[i] + f[i]);
+    const double t = lt[i] + fma(r[i] * r[i], p[i], r[i]);
     const double de = double(e[i]);
-    out[i] = fma(de, kLn2Hi, fma(de, kLn2Lo, lnm));
+    out[i] = fma(de, kLn2Hi, fma(de, kLn2Lo, t));
   }
 }
 
@@ -232,10 +224,10 @@ TWMC_HD inline void log_u32(const uint32_t* a, double* out, const Series& k = kS
 // Re = sqrt(-2 ln u1) cos(2 pi theta), Im = sqrt(-2 ln u1) sin(2 pi theta).
 template <int N>
 TWMC_HD inline void box_muller(const uint32_t* a, const uint32_t* b, double* re, double* im,
-                               const Series& k = kSeries) {
+                               const Series& k = kSeries, const double* tab = kLogTab) {
   TWMC_NO_CONTRACT
   double l[N], s[N], c[N];
-  log_u32<N>(a, l, k);
+  log_u32<N>(a, l, k, tab);
   sincos_turn32<N>(b, s, c, k);
   TWMC_UNROLL
   for (int i = 0; i < N; ++i) {

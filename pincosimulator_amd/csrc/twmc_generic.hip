@@ -35,15 +35,18 @@ __device__ cpx<T>* fft2d(cpx<T>* a, cpx<T>* b, const StepArgs& g, const cpx<T>* 
   return cur;
 }
 
+// FP64 ln table (rows c_j, L_j of twmc_f64math.h's log_u32)
+alignas(16) __device__ const double kLogTabDev[256] = TWMC_F64_LOG_TABLE;
+
 // the step's Wiener increments of a site pair from one Philox draw: FP64 in
 // the range-specialised forms of twmc_f64math.h, both sites side by side; FP32
 // on the hardware transcendental units
 template <typename T>
-__device__ __forceinline__ void box_muller_pair(const u32x4& x, T (&re)[2], T (&im)[2],
-                                                const f64::Series& k = f64::kSeries) {
+__device__ __forceinline__ void box_muller_pair(const u32x4& x, T (&re)[2], T (&im)[2], const f64::Series& k,
+                                                const double* ltab) {
   if constexpr (sizeof(T) == 8) {
     const uint32_t a[2] = {x.x, x.z}, b[2] = {x.y, x.w};
-    f64::box_muller<2>(a, b, re, im, k);
+    f64::box_muller<2>(a, b, re, im, k, ltab);
   } else {
     box_muller_f32(x.x, x.y, re[0], im[0]);
     box_muller_f32(x.z, x.w, re[1], im[1]);
@@ -93,6 +96,15 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
   // NN FP64: the Box-Muller series coefficients held in registers for the run
   f64::Series bmk = f64::kSeries;
   if constexpr (NN && sizeof(T) == 8) f64::pin(bmk);
+  // FP64 ln table: an LDS copy after the state for the nearest-neighbour
+  // launch, else read from device memory
+  const double* ltab = kLogTabDev;
+  if constexpr (NN && sizeof(T) == 8) {
+    double* lw = reinterpret_cast<double*>(bufA + n);
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) lw[i] = kLogTabDev[i];
+    ltab = lw;
+    __syncthreads();
+  }
   double* part = g.partials + (size_t)blockIdx.x * kQ * n;
   constexpr bool kStencil = sizeof(T) == 4 || NN;
   // x = floor(i / ny) as umulhi(i, ceil(2^32 / ny)): exact for i ny < 2^32
@@ -264,7 +276,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
         if (p < npairs) {
           u32x4 x = philox4x32_10(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
           T xr[2], xi[2];
-          box_muller_pair<T>(x, xr, xi, bmk);
+          box_muller_pair<T>(x, xr, xi, bmk, ltab);
           // P2: the pair's sites 2p, 2p + 1 share a row (ny even) and are each
           // other's y neighbours; their other y neighbours are rb + (y0 - 1) and
           // rb + (y0 + 2) (mod ny), their x neighbours adjacent pairs of sites.
@@ -596,8 +608,11 @@ const void* generic_nn_ptr(int precision, int ppt, bool unif, bool p2) {
 
 }  // namespace
 
-// LDS of a nearest-neighbour step launch: the per-wave sums and one copy of the state
-size_t generic_nn_lds_bytes(int n, int precision) { return 64 + (size_t)n * (precision == 64 ? 16 : 8); }
+// LDS of a nearest-neighbour step launch: the per-wave sums, one copy of the
+// state and, FP64, the ln table
+size_t generic_nn_lds_bytes(int n, int precision) {
+  return 64 + (size_t)n * (precision == 64 ? 16 : 8) + (precision == 64 ? sizeof(double) * 256 : 0);
+}
 
 hipError_t generic_nn_occupancy(int precision, int nx, int ny, size_t lds, bool site_uniform,
                                 int* blocks_per_cu) {
