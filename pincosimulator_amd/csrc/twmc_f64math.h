@@ -4,15 +4,19 @@
 // library log / sin / cos spend most of their instructions on argument ranges
 // that never occur here (large angles, subnormals, infinities); these forms
 // reduce exactly in integer arithmetic and evaluate short polynomials:
-//   sin, cos: the quarter turn nearest to b is removed from b itself, leaving
-//     an angle |x| <= pi/4; sin x = x - x^3 ps(x^2), cos x = 1 + x^2 pc(x^2);
+//   sin, cos: the 1/256 turn j nearest to b is removed from b itself; a
+//     256-row table (twmc_f64tab.h) holds cos and sin of theta_j, 2 pi j/256
+//     plus a tiny offset o_j that makes both nearly exact doubles; the rest,
+//     x = (b - j 2^24) 2 pi 2^-32 - o_j, |x| < pi/256 + 2^-40, enters by
+//     sin x = x - x^3 ps(x^2), cos x - 1 = x^2 pc(x^2) and the angle sum;
 //   ln: u1 = m 2^e from the exponent and significand bits of a + 1/2, m in
 //     [1, 2); a 128-row table (twmc_f64tab.h) indexed by m's top 7 fraction
 //     bits gives c_j and L_j with ln m = L_j + ln(1 + r), r = m c_j - 1
 //     exact and |r| <= 2^-7, ln(1 + r) = r + r^2 pq(r), plus e ln 2 with ln 2
 //     split so that e ln2_hi is exact;
-//   ps, pc, pq: degree-6 Chebyshev fits over those ranges (gen_f64_series.py),
-//     each adding less than 2e-17 to the relative error of its function;
+//   ps, pc (degree 2), pq (degree 6): Chebyshev fits over those ranges
+//     (gen_f64_series.py), each adding less than 1e-17 to the relative error
+//     of its function;
 //   the radius sqrt(-2 ln u1) by Newton steps from the hardware reciprocal
 //     square root: its argument (2e-10 < -2 ln u1 < 46) needs none of the
 //     library square root's scaling for extreme exponents.
@@ -105,22 +109,21 @@ TWMC_HD inline double sqrt_normal(double x) {
   return fma(fma(-g, g, x), h, g);
 }
 
-// The series coefficients, highest degree first: the top one, then the six
-// others (near the Taylor coefficients 1/15!, -1/14!, -1/8, ..., which the fit
-// adjusts).  A kernel may hand the functions a copy held in vector registers
-// for its whole run (pin), instead of having every use form its constant anew.
+// The series coefficients, highest degree first: the top one, then the
+// others (near the Taylor coefficients 1/7!, -1/5!, 1/3!; -1/6!, 1/4!, -1/2!;
+// -1/8, 1/7, ..., -1/2, which the fit adjusts).  A kernel may hand the
+// functions a copy held in vector registers for its whole run (pin), instead of
+// having every use form its constant anew.
 struct Series {
-  double s_top, ks[6];  // ps: ~1/15!, ~-1/13!, ..., 1/3!
-  double c_top, kc[6];  // pc: ~-1/14!, ~1/12!, ..., -1/2!
-  double l_top, kl[6];  // pq: ~-1/8, ~1/7, ..., -1/2
+  double s_top, ks[2];  // ps
+  double c_top, kc[2];  // pc
+  double l_top, kl[6];  // pq
 };
 constexpr Series kSeries = {
-    7.586697117706918e-13,
-    {-1.6058531618986147e-10, 2.5052106232447578e-08, -2.7557319219339167e-06, 0.00019841269841265065,
-     -0.008333333333333331, 0.16666666666666666},
-    -1.1367998654022494e-11,
-    {2.0875886738047052e-09, -2.7557315566341895e-07, 2.480158729369346e-05, -0.0013888888888880775,
-     0.04166666666666664, -0.5},
+    0.000198412075901208,
+    {-0.008333333333298177, 0.16666666666666666},
+    -0.001388883286287393,
+    {0.041666666666350265, -0.5},
     -0.1250106818314722,
     {0.14286901154505832, -0.1666666663406768, 0.1999999996377896, -0.2500000000000025, 0.3333333333333361,
      -0.5}};
@@ -131,31 +134,47 @@ constexpr Series kSeries = {
 TWMC_HD inline void pin(Series& k) {
 #if defined(__HIP_DEVICE_COMPILE__)
   asm volatile("" : "+v"(k.s_top), "+v"(k.c_top), "+v"(k.l_top));
-  for (int j = 0; j < 6; ++j) asm volatile("" : "+v"(k.ks[j]), "+v"(k.kc[j]), "+v"(k.kl[j]));
+  for (int j = 0; j < 2; ++j) asm volatile("" : "+v"(k.ks[j]), "+v"(k.kc[j]));
+  for (int j = 0; j < 6; ++j) asm volatile("" : "+v"(k.kl[j]));
 #else
   (void)k;
 #endif
 }
 
+// The tables (TWMC_F64_TABLES, 1024 doubles): the ln rows (c_j, L_j) from
+// kTabLog, the sin / cos rows (cos theta_j, sin theta_j) from kTabSinCos, the
+// offsets o_j from kTabOffset.  On the host the functions read them from
+// kTables; device code hands them a copy in device memory.
+constexpr int kTabLog = 0, kTabSinCos = 256, kTabOffset = 768, kTabSize = 1024;
+constexpr double kTables[kTabSize] = TWMC_F64_TABLES;
+
+// Two doubles read together: a table row.
+struct alignas(16) Row {
+  double a, b;
+};
+
 // sin and cos of 2 pi b[i] 2^-32, i < N.
 template <int N>
-TWMC_HD inline void sincos_turn32(const uint32_t* b, double* s, double* c, const Series& k = kSeries) {
+TWMC_HD inline void sincos_turn32(const uint32_t* b, double* s, double* c, const Series& k = kSeries,
+                                  const double* tab = kTables) {
   TWMC_NO_CONTRACT
-  double x[N], z[N], ps[N], pc[N];
-  uint32_t q[N];
+  const Row* rows = reinterpret_cast<const Row*>(tab + kTabSinCos);
+  double x[N], z[N], ps[N], pc[N], cj[N], sj[N];
   TWMC_UNROLL
   for (int i = 0; i < N; ++i) {
-    // nearest quarter turn q (mod 4) and the remainder b - q 2^30 in [-2^29, 2^29)
-    q[i] = (b[i] + 0x20000000u) >> 30;
-    const int32_t r = int32_t(b[i] - (q[i] << 30));
-    x[i] = double(r) * kTurn32;  // |x| <= pi/4
+    // b = j 2^24 + rem (mod 2^32): j the nearest 1/256 turn, rem in [-2^23, 2^23)
+    const uint32_t j = (b[i] + 0x800000u) >> 24;
+    const int32_t rem = int32_t(b[i] << 8) >> 8;
+    x[i] = fma(double(rem), kTurn32, -tab[kTabOffset + j]);  // angle - theta_j
     z[i] = x[i] * x[i];
+    const Row row = rows[j];
+    cj[i] = row.a;
+    sj[i] = row.b;
     ps[i] = k.s_top;
     pc[i] = k.c_top;
   }
-  // sin x = x - x z ps(z), cos x = 1 + z pc(z), z = x^2
   TWMC_UNROLL
-  for (int j = 0; j < 6; ++j)
+  for (int j = 0; j < 2; ++j)
     TWMC_UNROLL
     for (int i = 0; i < N; ++i) {
       ps[i] = fma3(ps[i], z[i], k.ks[j]);
@@ -163,33 +182,22 @@ TWMC_HD inline void sincos_turn32(const uint32_t* b, double* s, double* c, const
     }
   TWMC_UNROLL
   for (int i = 0; i < N; ++i) {
+    // sin x and cos x - 1, then the angle sum with theta_j
     const double sx = fma(-(x[i] * z[i]), ps[i], x[i]);
-    const double cx = fma(pc[i], z[i], 1.0);
-    // rotate by q quarter turns: (c, s) -> (-s, c) per quarter
-    const bool swap = (q[i] & 1u) != 0;
-    const double cs = swap ? sx : cx, sn = swap ? cx : sx;
-    c[i] = ((q[i] + 1u) & 2u) ? -cs : cs;
-    s[i] = (q[i] & 2u) ? -sn : sn;
+    const double cm = z[i] * pc[i];
+    c[i] = fma(cj[i], cm, fma(-sj[i], sx, cj[i]));
+    s[i] = fma(sj[i], cm, fma(cj[i], sx, sj[i]));
   }
 }
 
-// One row of the ln table: r = m c - 1 and ln m = L + ln(1 + r) (with the
-// exponent adjusted by one for m > sqrt 2, which L absorbs as -ln 2).
-struct alignas(16) LogRow {
-  double c, L;
-};
-
-// The table's 128 rows as 256 doubles (twmc_f64tab.h).  On the host the
-// functions read it from here; device code hands them a copy in device memory.
-constexpr double kLogTab[256] = TWMC_F64_LOG_TABLE;
-
-// ln((a[i] + 1/2) 2^-32), in (-22.9, 0), i < N; tab: the 256 doubles of
-// TWMC_F64_LOG_TABLE.
+// ln((a[i] + 1/2) 2^-32), in (-22.9, 0), i < N.  A row (c, L) of the ln
+// table: r = m c - 1 and ln m = L + ln(1 + r), the exponent raised by one for
+// m > sqrt 2, which L absorbs as -ln 2.
 template <int N>
 TWMC_HD inline void log_u32(const uint32_t* a, double* out, const Series& k = kSeries,
-                            const double* tab = kLogTab) {
+                            const double* tab = kTables) {
   TWMC_NO_CONTRACT
-  const LogRow* rows = reinterpret_cast<const LogRow*>(tab);
+  const Row* rows = reinterpret_cast<const Row*>(tab + kTabLog);
   double r[N], lt[N], p[N];
   int e[N];
   TWMC_UNROLL
@@ -202,9 +210,9 @@ TWMC_HD inline void log_u32(const uint32_t* a, double* out, const Series& k = kS
     // E - 1055, plus one for rows j >= 53: adding 75 to j carries into E
     e[i] = int((hi + (75u << 13)) >> 20) - 1055;
     const double m = from_bits((uint64_t((hi & 0xfffffu) | 0x3ff00000u) << 32) | (vb & 0xffffffffull));
-    const LogRow row = rows[j];
-    r[i] = fma(m, row.c, -1.0);  // exact: |r| <= 2^-7, m c has at most 52 bits
-    lt[i] = row.L;
+    const Row row = rows[j];
+    r[i] = fma(m, row.a, -1.0);  // exact: |r| <= 2^-7, m c has at most 52 bits
+    lt[i] = row.b;
     p[i] = k.l_top;
   }
   // ln(1 + r) = r + r^2 pq(r)
@@ -224,11 +232,11 @@ TWMC_HD inline void log_u32(const uint32_t* a, double* out, const Series& k = kS
 // Re = sqrt(-2 ln u1) cos(2 pi theta), Im = sqrt(-2 ln u1) sin(2 pi theta).
 template <int N>
 TWMC_HD inline void box_muller(const uint32_t* a, const uint32_t* b, double* re, double* im,
-                               const Series& k = kSeries, const double* tab = kLogTab) {
+                               const Series& k = kSeries, const double* tab = kTables) {
   TWMC_NO_CONTRACT
   double l[N], s[N], c[N];
   log_u32<N>(a, l, k, tab);
-  sincos_turn32<N>(b, s, c, k);
+  sincos_turn32<N>(b, s, c, k, tab);
   TWMC_UNROLL
   for (int i = 0; i < N; ++i) {
     const double r = sqrt_normal(-2.0 * l[i]);

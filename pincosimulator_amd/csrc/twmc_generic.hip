@@ -35,8 +35,8 @@ __device__ cpx<T>* fft2d(cpx<T>* a, cpx<T>* b, const StepArgs& g, const cpx<T>* 
   return cur;
 }
 
-// FP64 ln table (rows c_j, L_j of twmc_f64math.h's log_u32)
-alignas(16) __device__ const double kLogTabDev[256] = TWMC_F64_LOG_TABLE;
+// the FP64 Box-Muller's ln and sin / cos tables (twmc_f64math.h)
+alignas(16) __device__ const double kF64TabDev[f64::kTabSize] = TWMC_F64_TABLES;
 
 // the step's Wiener increments of a site pair from one Philox draw: FP64 in
 // the range-specialised forms of twmc_f64math.h, both sites side by side; FP32
@@ -96,12 +96,12 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
   // NN FP64: the Box-Muller series coefficients held in registers for the run
   f64::Series bmk = f64::kSeries;
   if constexpr (NN && sizeof(T) == 8) f64::pin(bmk);
-  // FP64 ln table: an LDS copy after the state for the nearest-neighbour
-  // launch, else read from device memory
-  const double* ltab = kLogTabDev;
+  // FP64 Box-Muller tables: an LDS copy after the state for the
+  // nearest-neighbour launch, else read from device memory
+  const double* ltab = kF64TabDev;
   if constexpr (NN && sizeof(T) == 8) {
     double* lw = reinterpret_cast<double*>(bufA + n);
-    for (int i = threadIdx.x; i < 256; i += blockDim.x) lw[i] = kLogTabDev[i];
+    for (int i = threadIdx.x; i < f64::kTabSize; i += blockDim.x) lw[i] = kF64TabDev[i];
     ltab = lw;
     __syncthreads();
   }
@@ -609,9 +609,9 @@ const void* generic_nn_ptr(int precision, int ppt, bool unif, bool p2) {
 }  // namespace
 
 // LDS of a nearest-neighbour step launch: the per-wave sums, one copy of the
-// state and, FP64, the ln table
+// state and, FP64, the Box-Muller tables
 size_t generic_nn_lds_bytes(int n, int precision) {
-  return 64 + (size_t)n * (precision == 64 ? 16 : 8) + (precision == 64 ? sizeof(double) * 256 : 0);
+  return 64 + (size_t)n * (precision == 64 ? 16 : 8) + (precision == 64 ? sizeof(double) * f64::kTabSize : 0);
 }
 
 hipError_t generic_nn_occupancy(int precision, int nx, int ny, size_t lds, bool site_uniform,

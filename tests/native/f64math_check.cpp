@@ -5,9 +5,10 @@
 // and exits non-zero when a bound is exceeded or a value is not finite.
 // Inputs: every 32-bit a / b of the form k * 4093 + (k * 2654435761 mod 4093)
 // for k below 2^20 (1M values spread over the whole range), plus the edges
-// 0, 1, 2^32 - 1, every power of two, and the quarter- and eighth-turn
-// boundaries of b with their neighbours, and a + 1/2 at and just above
-// sqrt(2) 2^p, where ln's reduction switches.
+// 0, 1, 2^32 - 1, every power of two, the eighth turns, the sin / cos
+// table's 1/256 turns and the midpoints between them (where the row switches)
+// with their neighbours, a + 1/2 at and just above sqrt(2) 2^p, and at the ln
+// table's row boundaries (1 + j/128) 2^p.
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -35,8 +36,18 @@ int main() {
     xs.push_back(v - 1u);
     xs.push_back(v + 1u);
   }
-  for (uint32_t j = 0; j < 8; ++j)  // eighth turns: the reduction's boundaries
+  for (uint32_t j = 0; j < 8; ++j)  // eighth turns
     for (int d = -2; d <= 2; ++d) xs.push_back(j * 0x20000000u + (uint32_t)d);
+  for (uint32_t j = 0; j < 256; ++j)  // the sin / cos table's rows and the midpoints between them
+    for (int d = -2; d <= 2; ++d) {
+      xs.push_back(j * 0x1000000u + (uint32_t)d);
+      xs.push_back(j * 0x1000000u + 0x800000u + (uint32_t)d);
+    }
+  for (uint32_t j = 0; j < 128; ++j)  // the ln table's row boundaries: a + 1/2 = (1 + j/128) 2^p
+    for (int p = 8; p < 32; p += 3) {
+      const uint32_t c = (uint32_t)((1.0 + j / 128.0) * (double)(1u << p));
+      for (int d = -1; d <= 1; ++d) xs.push_back(c + (uint32_t)d);
+    }
   for (int p = 1; p < 32; ++p) {  // ln's reduction boundary: a + 1/2 near sqrt(2) 2^p
     const uint32_t c = (uint32_t)(1.4142135623730951 * (double)(1u << p));
     for (int d = -3; d <= 3; ++d) xs.push_back(c + (uint32_t)d);

@@ -1,10 +1,11 @@
 """Accuracy of the FP64 Box-Muller the FP64 step kernels use
 (pincosimulator_amd/csrc/twmc_f64math.h), built for the host with g++ and
 checked against x87 long double references over ~1M arguments spread over the
-whole 32-bit range plus the reduction's boundaries: ln within 2 ulp, sin / cos
-of 2 pi b 2^-32 within 2 ulp (absolute, in ulps of 1/2, near their zeros),
-the complex sample within 4 ulp of its radius.  A dropped series term, a wrong
-quadrant or a wrong ln 2 split moves these errors by orders of magnitude."""
+whole 32-bit range plus the reductions' boundaries (every row switch of the ln
+and sin / cos tables): ln within 2 ulp, sin / cos of 2 pi b 2^-32 within 2 ulp
+(absolute, in ulps of 1/2, near their zeros), the complex sample within 4 ulp
+of its radius.  A dropped series term, a wrong table row or entry, or a wrong
+ln 2 split moves these errors by orders of magnitude."""
 import os
 import subprocess
 
