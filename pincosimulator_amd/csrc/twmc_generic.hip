@@ -142,13 +142,20 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
     const int iters = g.n_steps > 0 ? g.n_steps : (g.capture ? 1 : 0);
     int ft_id = 0;
     for (int s = 0; s < iters; ++s) {
-      // stage beta in LDS
+      // stage beta in LDS.  P2: even sites in the first half, odd sites in the
+      // second (site 2p at p, 2p + 1 at n/2 + p), so a wave's stores and
+      // neighbour reads step 16 bytes from lane to lane (no bank conflicts)
 #pragma unroll
       for (int j = 0; j < PPT; ++j) {
         int p = threadIdx.x + kBlock * j;
         if (p < npairs) {
-          bufA[2 * p] = beta[2 * j];
-          if (2 * p + 1 < n) bufA[2 * p + 1] = beta[2 * j + 1];
+          if constexpr (P2) {
+            bufA[p] = beta[2 * j];
+            bufA[npairs + p] = beta[2 * j + 1];
+          } else {
+            bufA[2 * p] = beta[2 * j];
+            if (2 * p + 1 < n) bufA[2 * p + 1] = beta[2 * j + 1];
+          }
         }
       }
       __syncthreads();
@@ -278,18 +285,22 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
           T xr[2], xi[2];
           box_muller_pair<T>(x, xr, xi, bmk, ltab);
           // P2: the pair's sites 2p, 2p + 1 share a row (ny even) and are each
-          // other's y neighbours; their other y neighbours are rb + (y0 - 1) and
-          // rb + (y0 + 2) (mod ny), their x neighbours adjacent pairs of sites.
+          // other's y neighbours; their other y neighbours and their x
+          // neighbours are read from the split LDS layout.
           // Both sites' new values are formed before either is stored, so beta
           // holds the pair's state before this step throughout (P2).
           cpx<T> znew[2];
           int nbl = 0, nbr = 0, nbu = 0, nbd = 0;
           if constexpr (P2) {
-            const int i0 = 2 * p + opq, ym = g.ny - 1, y0 = i0 & ym, rb = i0 - y0, nm = n - 1;
-            nbl = rb + ((y0 - 1) & ym);
-            nbr = rb + ((y0 + 2) & ym);
-            nbu = (i0 + g.ny) & nm;
-            nbd = (i0 - g.ny) & nm;
+            // in pairs (yh pairs per row): the left neighbour of 2p is the odd
+            // site of pair rb + q - 1, the right neighbour of 2p + 1 the even
+            // site of pair rb + q + 1 (q the pair's place in its row, mod yh);
+            // the x neighbours are the pairs p +- yh (mod n/2), both sites
+            const int pp = p + opq, yh = g.ny >> 1, q = pp & (yh - 1), rb = pp - q, hm = npairs - 1;
+            nbl = npairs + rb + ((q - 1) & (yh - 1));
+            nbr = rb + ((q + 1) & (yh - 1));
+            nbu = (pp + yh) & hm;
+            nbd = (pp - yh) & hm;
           }
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
@@ -309,8 +320,8 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
                     sr = a.re + c.re;
                     si = a.im + c.im;
                     if (g.nx > 1) {
-                      const cpx<T> d = bufA[nbu + h];
-                      const cpx<T> e = bufA[nbd + h];
+                      const cpx<T> d = bufA[h * npairs + nbu];
+                      const cpx<T> e = bufA[h * npairs + nbd];
                       sr += d.re + e.re;
                       si += d.im + e.im;
                     }
