@@ -5,9 +5,8 @@
 // that never occur here (large angles, subnormals, infinities); these forms
 // reduce exactly in integer arithmetic and evaluate short polynomials:
 //   sin, cos: the 1/256 turn j nearest to b is removed from b itself; a
-//     256-row table (twmc_f64tab.h) holds cos and sin of theta_j, 2 pi j/256
-//     plus a tiny offset o_j that makes both nearly exact doubles; the rest,
-//     x = (b - j 2^24) 2 pi 2^-32 - o_j, |x| < pi/256 + 2^-40, enters by
+//     256-row table (twmc_f64tab.h) holds cos and sin of theta_j = 2 pi j/256;
+//     the rest, x = (b - j 2^24) 2 pi 2^-32, |x| <= pi/256, enters by
 //     sin x = x - x^3 ps(x^2), cos x - 1 = x^2 pc(x^2) and the angle sum;
 //   ln: u1 = m 2^e from the exponent and significand bits of a + 1/2, m in
 //     [1, 2); a 128-row table (twmc_f64tab.h) indexed by m's top 7 fraction
@@ -120,9 +119,9 @@ struct Series {
   double l_top, kl[6];  // pq
 };
 constexpr Series kSeries = {
-    0.000198412075901208,
+    0.0001984120759012081,
     {-0.008333333333298177, 0.16666666666666666},
-    -0.001388883286287393,
+    -0.0013888832862873938,
     {0.041666666666350265, -0.5},
     -0.1250106818314722,
     {0.14286901154505832, -0.1666666663406768, 0.1999999996377896, -0.2500000000000025, 0.3333333333333361,
@@ -141,11 +140,11 @@ TWMC_HD inline void pin(Series& k) {
 #endif
 }
 
-// The tables (TWMC_F64_TABLES, 1024 doubles): the ln rows (c_j, L_j) from
-// kTabLog, the sin / cos rows (cos theta_j, sin theta_j) from kTabSinCos, the
-// offsets o_j from kTabOffset.  On the host the functions read them from
-// kTables; device code hands them a copy in device memory.
-constexpr int kTabLog = 0, kTabSinCos = 256, kTabOffset = 768, kTabSize = 1024;
+// The tables (TWMC_F64_TABLES, 768 doubles): the ln rows (c_j, L_j) from
+// kTabLog, the sin / cos rows (cos theta_j, sin theta_j) from kTabSinCos.  On
+// the host the functions read them from kTables; device code hands them a copy
+// in device memory.
+constexpr int kTabLog = 0, kTabSinCos = 256, kTabSize = 768;
 constexpr double kTables[kTabSize] = TWMC_F64_TABLES;
 
 // Two doubles read together: a table row.
@@ -165,7 +164,7 @@ TWMC_HD inline void sincos_turn32(const uint32_t* b, double* s, double* c, const
     // b = j 2^24 + rem (mod 2^32): j the nearest 1/256 turn, rem in [-2^23, 2^23)
     const uint32_t j = (b[i] + 0x800000u) >> 24;
     const int32_t rem = int32_t(b[i] << 8) >> 8;
-    x[i] = fma(double(rem), kTurn32, -tab[kTabOffset + j]);  // angle - theta_j
+    x[i] = double(rem) * kTurn32;  // angle - theta_j
     z[i] = x[i] * x[i];
     const Row row = rows[j];
     cj[i] = row.a;
