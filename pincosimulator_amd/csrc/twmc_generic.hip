@@ -66,8 +66,11 @@ __device__ __forceinline__ double warp_sum(double v) {
 // first entries are read once, into registers, instead of per site and step.
 // P2 (with NN): ny a power of two >= 2 and nx = 1 or a power of two >= 2, so
 // the neighbour indices are masks instead of a division and wrap tests.
-template <typename T, int PPT, bool NN = false, bool UNIF = false, bool P2 = false>
-__global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
+// BLOCK: threads per workgroup (kBlock, or 1024 for the FP64 site-uniform
+// step launch of 2049 to 4096 sites: two pairs per thread fit 128 registers,
+// four waves per SIMD in the one workgroup per CU that the LDS state allows).
+template <typename T, int PPT, bool NN = false, bool UNIF = false, bool P2 = false, int BLOCK = kBlock>
+__global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   static_assert(!UNIF || NN, "site-uniform launches are nearest-neighbour launches");
   static_assert(!P2 || UNIF, "the mask form is instantiated for site-uniform launches");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -132,7 +135,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
     cpx<T> beta[2 * PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-      int p = threadIdx.x + kBlock * j;
+      int p = threadIdx.x + BLOCK * j;
       if (p < npairs) {
         beta[2 * j] = gbeta[2 * p];
         if (2 * p + 1 < n) beta[2 * j + 1] = gbeta[2 * p + 1];
@@ -147,7 +150,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
       // neighbour reads step 16 bytes from lane to lane (no bank conflicts)
 #pragma unroll
       for (int j = 0; j < PPT; ++j) {
-        int p = threadIdx.x + kBlock * j;
+        int p = threadIdx.x + BLOCK * j;
         if (p < npairs) {
           if constexpr (P2) {
             bufA[p] = beta[2 * j];
@@ -179,7 +182,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
         double m2 = 0.0, m4 = 0.0;
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
-          int p = threadIdx.x + kBlock * j;
+          int p = threadIdx.x + BLOCK * j;
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             int i = 2 * p + h;
@@ -226,7 +229,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
           // the transform overwrote bufA: stage beta again for the neighbour sums
 #pragma unroll
           for (int j = 0; j < PPT; ++j) {
-            int p = threadIdx.x + kBlock * j;
+            int p = threadIdx.x + BLOCK * j;
             if (p < npairs) {
               bufA[2 * p] = beta[2 * j];
               if (2 * p + 1 < n) bufA[2 * p + 1] = beta[2 * j + 1];
@@ -279,7 +282,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
       }
 #pragma unroll
       for (int j = 0; j < PPT; ++j) {
-        int p = threadIdx.x + kBlock * j;
+        int p = threadIdx.x + BLOCK * j;
         if (p < npairs) {
           u32x4 x = philox4x32_10(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
           T xr[2], xi[2];
@@ -388,7 +391,7 @@ __global__ void __launch_bounds__(kBlock) generic_step_kernel(StepArgs g) {
 
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-      int p = threadIdx.x + kBlock * j;
+      int p = threadIdx.x + BLOCK * j;
       if (p < npairs) {
         gbeta[2 * p] = beta[2 * j];
         if (2 * p + 1 < n) gbeta[2 * p + 1] = beta[2 * j + 1];
@@ -594,7 +597,9 @@ hipError_t launch_generic_p(const StepArgs& a, int grid, int block, size_t lds, 
 }
 
 template <typename T, bool UNIF, bool P2>
-const void* generic_nn_ptr_u(int ppt) {
+const void* generic_nn_ptr_u(int ppt, int block) {
+  if constexpr (sizeof(T) == 8 && UNIF)
+    if (block == 1024) return (const void*)generic_step_kernel<T, 2, true, UNIF, P2, 1024>;
   return ppt == 1   ? (const void*)generic_step_kernel<T, 1, true, UNIF, P2>
          : ppt == 2 ? (const void*)generic_step_kernel<T, 2, true, UNIF, P2>
          : ppt == 8 ? (const void*)generic_step_kernel<T, 8, true, UNIF, P2>
@@ -608,13 +613,20 @@ bool pow2_lattice(int nx, int ny) {
   return p2(ny) && (nx == 1 || p2(nx));
 }
 
-const void* generic_nn_ptr(int precision, int ppt, bool unif, bool p2) {
+const void* generic_nn_ptr(int precision, int ppt, bool unif, bool p2, int block) {
   if (precision == 64) {
-    if (unif) return p2 ? generic_nn_ptr_u<double, true, true>(ppt) : generic_nn_ptr_u<double, true, false>(ppt);
-    return generic_nn_ptr_u<double, false, false>(ppt);
+    if (unif)
+      return p2 ? generic_nn_ptr_u<double, true, true>(ppt, block) : generic_nn_ptr_u<double, true, false>(ppt, block);
+    return generic_nn_ptr_u<double, false, false>(ppt, block);
   }
-  if (unif) return p2 ? generic_nn_ptr_u<float, true, true>(ppt) : generic_nn_ptr_u<float, true, false>(ppt);
-  return generic_nn_ptr_u<float, false, false>(ppt);
+  if (unif)
+    return p2 ? generic_nn_ptr_u<float, true, true>(ppt, block) : generic_nn_ptr_u<float, true, false>(ppt, block);
+  return generic_nn_ptr_u<float, false, false>(ppt, block);
+}
+
+// threads per workgroup of a nearest-neighbour step launch
+int generic_nn_block(int precision, int n, bool unif) {
+  return (precision == 64 && unif && generic_pairs_per_thread(n, 1024) == 2) ? 1024 : kBlock;
 }
 
 }  // namespace
@@ -627,24 +639,26 @@ size_t generic_nn_lds_bytes(int n, int precision) {
 
 hipError_t generic_nn_occupancy(int precision, int nx, int ny, size_t lds, bool site_uniform,
                                 int* blocks_per_cu) {
-  const int ppt = generic_pairs_per_thread(nx * ny, kBlock);
-  const void* f = generic_nn_ptr(precision, ppt, site_uniform, pow2_lattice(nx, ny));
+  const int block = generic_nn_block(precision, nx * ny, site_uniform);
+  const int ppt = generic_pairs_per_thread(nx * ny, block);
+  const void* f = generic_nn_ptr(precision, ppt, site_uniform, pow2_lattice(nx, ny), block);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, kBlock, lds);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, block, lds);
 }
 
 hipError_t launch_generic_nn(const StepArgs& a, int precision, int grid, size_t lds, hipStream_t s) {
-  const int ppt = generic_pairs_per_thread(a.n, kBlock);
-  const void* f = generic_nn_ptr(precision, ppt, a.site_uniform != 0, pow2_lattice(a.nx, a.ny));
+  const int block = generic_nn_block(precision, a.n, a.site_uniform != 0);
+  const int ppt = generic_pairs_per_thread(a.n, block);
+  const void* f = generic_nn_ptr(precision, ppt, a.site_uniform != 0, pow2_lattice(a.nx, a.ny), block);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
   void* args[] = {const_cast<StepArgs*>(&a)};
-  return hipLaunchKernel(f, dim3(grid), dim3(kBlock), args, lds, s);
+  return hipLaunchKernel(f, dim3(grid), dim3(block), args, lds, s);
 }
 
 int generic_pairs_per_thread(int n, int block) {
