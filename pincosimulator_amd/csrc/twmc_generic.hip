@@ -125,11 +125,13 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
     const cpx<T>* Fp = reinterpret_cast<const cpx<T>*>(g.F) + poff;
     // UNIF: the site constants pre-scaled by dt, dt rsl, dt U and dt i F
     cpx<T> rd = mk(T(0), T(0)), ud = rd, fd = rd;
+    T rd1 = T(1);
     if constexpr (UNIF) {
       const cpx<T> r0 = rsl[0], u0 = Up[0], f0 = Fp[0];
       rd = mk(dt * r0.re, dt * r0.im);
       ud = mk(dt * u0.re, dt * u0.im);
       fd = mk(-dt * f0.im, dt * f0.re);
+      rd1 = T(1) + rd.re;
     }
 
     cpx<T> beta[2 * PPT];
@@ -284,7 +286,8 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
       for (int j = 0; j < PPT; ++j) {
         int p = threadIdx.x + BLOCK * j;
         if (p < npairs) {
-          u32x4 x = philox4x32_10(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
+          // the trajectory (one per workgroup) and step are wave-uniform
+          u32x4 x = philox4x32_10_wave(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
           T xr[2], xi[2];
           box_muller_pair<T>(x, xr, xi, bmk, ltab);
           // P2: the pair's sites 2p, 2p + 1 share a row (ny even) and are each
@@ -292,7 +295,8 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
           // neighbours are read from the split LDS layout.
           // Both sites' new values are formed before either is stored, so beta
           // holds the pair's state before this step throughout (P2).
-          cpx<T> znew[2];
+          cpx<T> znew[2], dz[2];
+          T c1[2];
           int nbl = 0, nbr = 0, nbu = 0, nbd = 0;
           if constexpr (P2) {
             // in pairs (yh pairs per row): the left neighbour of 2p is the odd
@@ -354,16 +358,37 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
                 // increment dt (c z + h + i F) + dt4 xi as one FMA chain per
                 // component, added to beta once; dt h = dt J (S_im, -S_re)
                 const T q = fma(z.re, z.re, fma(z.im, z.im, T(-1)));
-                const T cr = fma(-ud.im, q, rd.re), ci = fma(ud.re, q, rd.im);
-                T dr = fma(dt4, xr[h], fd.re);
-                dr = fma(djv, si, dr);
-                dr = fma(cr, z.re, dr);
-                dr = fma(-ci, z.im, dr);
-                T di = fma(dt4, xi[h], fd.im);
-                di = fma(-djv, sr, di);
-                di = fma(cr, z.im, di);
-                di = fma(ci, z.re, di);
-                z = mk(z.re + dr, z.im + di);
+                const T ci = fma(ud.re, q, rd.im);
+                if constexpr (sizeof(T) == 8) {
+                  // FP64: z + c z = (1 + c) z with 1 + rd.re formed once (its
+                  // rounding, 2^-53, is 1e-13 of dt rsl), the (1 + c) z term
+                  // last, so the new value still takes one rounding at |z|
+                  const T cr1 = fma(-ud.im, q, rd1);
+                  T dr = fma(dt4, xr[h], fd.re);
+                  dr = fma(djv, si, dr);
+                  dr = fma(-ci, z.im, dr);
+                  T di = fma(dt4, xi[h], fd.im);
+                  di = fma(-djv, sr, di);
+                  di = fma(ci, z.re, di);
+                  if constexpr (P2) {
+                    // applied after the pair's other site has read this one
+                    c1[h] = cr1;
+                    dz[h] = mk(dr, di);
+                  } else {
+                    z = mk(f64::fma3(cr1, z.re, dr), f64::fma3(cr1, z.im, di));
+                  }
+                } else {
+                  const T cr = fma(-ud.im, q, rd.re);
+                  T dr = fma(dt4, xr[h], fd.re);
+                  dr = fma(djv, si, dr);
+                  dr = fma(cr, z.re, dr);
+                  dr = fma(-ci, z.im, dr);
+                  T di = fma(dt4, xi[h], fd.im);
+                  di = fma(-djv, sr, di);
+                  di = fma(cr, z.im, di);
+                  di = fma(ci, z.re, di);
+                  z = mk(z.re + dr, z.im + di);
+                }
               } else {
                 cpx<T> r = rsl[i], u = Up[i], f = fv[i];
                 if (fs) f = mk(f.re + fdt * fs[i].re, f.im + fdt * fs[i].im);
@@ -381,8 +406,17 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
             }
           }
           if constexpr (P2) {
-            beta[2 * j] = znew[0];
-            beta[2 * j + 1] = znew[1];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              if constexpr (sizeof(T) == 8) {
+                // three-operand FMAs: the new value can take the old one's
+                // registers (the accumulator form would need a copy)
+                const cpx<T> z = beta[2 * j + h];
+                beta[2 * j + h] = mk(f64::fma3(c1[h], z.re, dz[h].re), f64::fma3(c1[h], z.im, dz[h].im));
+              } else {
+                beta[2 * j + h] = znew[h];
+              }
+            }
           }
         }
       }

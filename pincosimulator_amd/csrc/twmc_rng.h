@@ -76,6 +76,46 @@ __host__ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, ui
   return c;
 }
 
+// philox4x32_10 for counters whose step (c.x) and trajectory words (c.z, c.w)
+// are the same across the wave and only the pair (c.y) differs per lane, as
+// make_counter builds them in a kernel that steps one trajectory per wave or
+// workgroup.  The same rounds, with each XOR of round 2 and 3 grouped so that
+// its two wave-uniform operands combine first on the scalar unit: a vector
+// three-input XOR can read only one scalar operand, and would need a copy.
+// Which words are uniform, by round (U uniform, V per lane):
+//   in:  x U, y V, z U, w U      after 0: x V, y U, z U, w U
+//   after 1: x U, y U, z V, w V  after 2: x V, y V, z V, w U  (then all V)
+__host__ __device__ __forceinline__ u32x4 philox4x32_10_wave(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint32_t hi0, lo0, hi1, lo1;
+    mulhilo32(0xD2511F53u, c.x, hi0, lo0);
+    mulhilo32(0xCD9E8D57u, c.z, hi1, lo1);
+    u32x4 n;
+    if (r < 2) {
+      n.x = hi1 ^ c.y ^ k0;
+      n.z = hi0 ^ c.w ^ k1;
+    } else if (r == 2) {
+      n.x = hi1 ^ (c.y ^ k0);  // c.y, k0 uniform
+      n.z = c.w ^ (hi0 ^ k1);  // hi0, k1 uniform
+    } else if (r == 3) {
+      n.x = xor3(hi1, c.y, k0);
+      n.z = hi0 ^ (c.w ^ k1);  // c.w, k1 uniform
+    } else {
+      n.x = xor3(hi1, c.y, k0);
+      n.z = xor3(hi0, c.w, k1);
+    }
+    n.y = lo1;
+    n.w = lo0;
+    c = n;
+  }
+  return c;
+}
+
 // Round r (0-based) of philox4x32_10 alone, with that round's key
 // (k0 + r 0x9E3779B9, k1 + r 0xBB67AE85), for callers that spread the ten
 // rounds over other work; rounds 0..9 in order give philox4x32_10 exactly.
