@@ -486,7 +486,7 @@ int step_geometry(Ctx* c, size_t B, Geometry& geo) {
     HIPCHK(c, generic_occupancy(c->precision, c->n, geo.block, geo.lds, &per_cu));
     if (c->p.hopping == TWMC_HOP_FFT) {
       int per_cu_nn = 0;
-      geo.lds_nn = generic_nn_lds_bytes(c->n, c->precision);
+      geo.lds_nn = generic_nn_lds_bytes(c->p.nx, c->p.ny, c->precision, site_uniform(c));
       HIPCHK(c, generic_nn_occupancy(c->precision, c->p.nx, c->p.ny, geo.lds_nn, site_uniform(c), &per_cu_nn));
       if (per_cu_nn < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");
       geo.grid_nn = (int)std::min<size_t>(B, (size_t)per_cu_nn * cus);

@@ -101,9 +101,14 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   if constexpr (NN && sizeof(T) == 8) f64::pin(bmk);
   // FP64 Box-Muller tables: an LDS copy after the state for the
   // nearest-neighbour launch, else read from device memory
+  // DB (the 1024-thread FP64 launch on a power-of-two lattice, one workgroup
+  // per CU): two LDS copies of the state, each step reading one and writing
+  // the new values into the other, so one barrier per step instead of two
+  constexpr bool kDB = NN && P2 && BLOCK == 1024;
+  cpx<T>* const bufD = bufA + n;
   const double* ltab = kF64TabDev;
   if constexpr (NN && sizeof(T) == 8) {
-    double* lw = reinterpret_cast<double*>(bufA + n);
+    double* lw = reinterpret_cast<double*>(bufA + (kDB ? 2 * n : n));
     for (int i = threadIdx.x; i < f64::kTabSize; i += blockDim.x) lw[i] = kF64TabDev[i];
     ltab = lw;
     __syncthreads();
@@ -146,12 +151,26 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
 
     const int iters = g.n_steps > 0 ? g.n_steps : (g.capture ? 1 : 0);
     int ft_id = 0;
+    if constexpr (kDB) {
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) {
+        const int p = threadIdx.x + BLOCK * j;
+        if (p < npairs) {
+          bufA[p] = beta[2 * j];
+          bufA[npairs + p] = beta[2 * j + 1];
+        }
+      }
+      __syncthreads();
+    }
     for (int s = 0; s < iters; ++s) {
       // stage beta in LDS.  P2: even sites in the first half, odd sites in the
       // second (site 2p at p, 2p + 1 at n/2 + p), so a wave's stores and
-      // neighbour reads step 16 bytes from lane to lane (no bank conflicts)
+      // neighbour reads step 16 bytes from lane to lane (no bank conflicts).
+      // DB: the state of this step is already in nbuf, the new one goes to nxt
+      cpx<T>* const nbuf = (kDB && (s & 1)) ? bufD : bufA;
+      cpx<T>* const nxt = (kDB && (s & 1)) ? bufA : bufD;
 #pragma unroll
-      for (int j = 0; j < PPT; ++j) {
+      for (int j = 0; j < PPT && !kDB; ++j) {
         int p = threadIdx.x + BLOCK * j;
         if (p < npairs) {
           if constexpr (P2) {
@@ -163,7 +182,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
           }
         }
       }
-      __syncthreads();
+      if constexpr (!kDB) __syncthreads();
       cpx<T>* cur = bufA;
       const bool cap = !NN && (s == 0 && g.capture);
       // FP32 (and the FP64 step launch, NN) uses the FFT form's exact
@@ -322,13 +341,13 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
                   // (a dimension of length 1 has none): IDFT(K DFT beta)/N exactly
                   if constexpr (P2) {
                     // y + 1 and y - 1 neighbours, then x + 1 and x - 1
-                    const cpx<T> a = h == 0 ? beta[2 * j + 1] : bufA[nbr];
-                    const cpx<T> c = h == 0 ? bufA[nbl] : beta[2 * j];
+                    const cpx<T> a = h == 0 ? beta[2 * j + 1] : nbuf[nbr];
+                    const cpx<T> c = h == 0 ? nbuf[nbl] : beta[2 * j];
                     sr = a.re + c.re;
                     si = a.im + c.im;
                     if (g.nx > 1) {
-                      const cpx<T> d = bufA[h * npairs + nbu];
-                      const cpx<T> e = bufA[h * npairs + nbd];
+                      const cpx<T> d = nbuf[h * npairs + nbu];
+                      const cpx<T> e = nbuf[h * npairs + nbd];
                       sr += d.re + e.re;
                       si += d.im + e.im;
                     }
@@ -416,6 +435,10 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
               } else {
                 beta[2 * j + h] = znew[h];
               }
+            }
+            if constexpr (kDB) {
+              nxt[p] = beta[2 * j];
+              nxt[npairs + p] = beta[2 * j + 1];
             }
           }
         }
@@ -666,9 +689,13 @@ int generic_nn_block(int precision, int n, bool unif) {
 }  // namespace
 
 // LDS of a nearest-neighbour step launch: the per-wave sums, one copy of the
-// state and, FP64, the Box-Muller tables
-size_t generic_nn_lds_bytes(int n, int precision) {
-  return 64 + (size_t)n * (precision == 64 ? 16 : 8) + (precision == 64 ? sizeof(double) * f64::kTabSize : 0);
+// state (two for the double-buffered 1024-thread FP64 launch) and, FP64, the
+// Box-Muller tables
+size_t generic_nn_lds_bytes(int nx, int ny, int precision, bool site_uniform) {
+  const int n = nx * ny;
+  const bool db = generic_nn_block(precision, n, site_uniform) == 1024 && pow2_lattice(nx, ny);
+  return 64 + (size_t)n * (precision == 64 ? 16 : 8) * (db ? 2 : 1) +
+         (precision == 64 ? sizeof(double) * f64::kTabSize : 0);
 }
 
 hipError_t generic_nn_occupancy(int precision, int nx, int ny, size_t lds, bool site_uniform,

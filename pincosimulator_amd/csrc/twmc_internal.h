@@ -177,7 +177,7 @@ size_t generic_lds_bytes(int nx, int ny, int precision);
 int generic_pairs_per_thread(int n, int block);
 // the LDS kernel as a step launch with the hopping in its nearest-neighbour
 // form and no capture (FFT hopping): one LDS copy of the state
-size_t generic_nn_lds_bytes(int n, int precision);
+size_t generic_nn_lds_bytes(int nx, int ny, int precision, bool site_uniform);
 hipError_t generic_nn_occupancy(int precision, int nx, int ny, size_t lds, bool site_uniform,
                                 int* blocks_per_cu);
 hipError_t launch_generic_nn(const StepArgs& a, int precision, int grid, size_t lds, hipStream_t s);

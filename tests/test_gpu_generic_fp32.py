@@ -38,9 +38,12 @@ def test_fp32_nearest_neighbour_matches_fft_oracle(nx, ny):
 
 
 # power-of-two lattices (ny >= 2, nx = 1 or >= 2) take the step launch whose
-# neighbour indices are masks; (8, 6) and (32, 1) the division form
+# neighbour indices are masks; (8, 6) and (32, 1) the division form; 2049 to
+# 4096 sites the 1024-thread launch (double-buffered state on power-of-two
+# lattices: (4, 1024), (128, 32), (32, 128); (48, 48) single-buffered)
 @pytest.mark.parametrize("nx,ny", [(1, 7), (5, 7), (1, 127), (7, 11), (13, 14),
-                                   (1, 2), (1, 32), (2, 8), (4, 2), (16, 4), (8, 6), (32, 1)])
+                                   (1, 2), (1, 32), (2, 8), (4, 2), (16, 4), (8, 6), (32, 1),
+                                   (4, 1024), (128, 32), (32, 128), (48, 48)])
 def test_fp64_awkward_and_power_of_two_shapes_match_oracle(nx, ny):
     spec = dict(nx=nx, ny=ny, dt=0.005, gamma=1.0, J=1.3, omega=0.4, U=0.1, F=1.1, beta_init=0.2,
                 beta_sigma=0.5, frame_steps=50, seed=516, t_start=0.0, t_end=0.5)

@@ -100,6 +100,29 @@ def test_square_fp64(nx, ny):
     assert rel_err(res.traj, frames) < TOL64
 
 
+@pytest.mark.parametrize("nx,ny,n_traj", [(64, 64, 600), (8, 8, 5000), (48, 48, 600)])
+def test_fp64_more_trajectories_than_workgroups(nx, ny, n_traj):
+    """The persistent step launch: more trajectories than workgroups, so
+    workgroups step a second and third trajectory after the first (the
+    64x64 launch, 1024 threads and two LDS copies of the state per workgroup,
+    one workgroup per CU; 8x8, 256 threads; 48x48, 1024 threads, one copy).
+    Trajectories from the first, second and last round of the grid against the
+    oracle."""
+    spec = dict(nx=nx, ny=ny, dt=0.002, gamma=1.0, J=1.0, U=0.05, omega=1.0, F=2.0, beta_init=0.0,
+                beta_sigma=0.5, frame_steps=20, seed=2025, t_start=0.0, t_end=0.08)
+    ids = np.arange(n_traj, dtype=np.uint64) * 3 + 11
+    print(f"fp64 {nx}x{ny}, {n_traj} trajectories", flush=True)
+    eng = Engine(system(spec), precision=64, kernel="generic")
+    res = eng.run(ids, traj=True)
+    eng.close()
+    assert res.traj.shape == (n_traj, 3, nx * ny)
+    pick = [0, 1, 255, 256, 300, n_traj - 1]
+    ref = np.array([oracle.run_trajectory(dict(spec, lattice=oracle.LAT_FFT), int(ids[j]))[0] for j in pick])
+    err = rel_err(res.traj[pick], ref)
+    print(f"  max relative deviation {err:.2e}")
+    assert err < TOL64
+
+
 def test_fp64_64x64_subset():
     spec = dict(nx=64, ny=64, dt=0.002, gamma=1.0, J=1.0, U=0.05, omega=1.0, F=2.0, beta_init=0.0,
                 beta_sigma=0.5, frame_steps=500, seed=2024, t_start=0.0, t_end=2.0)
