@@ -10,7 +10,7 @@
 //     sin x = x - x^3 ps(x^2), cos x - 1 = x^2 pc(x^2) and the angle sum;
 //   ln: u1 = m 2^e from the exponent and significand bits of a + 1/2, m in
 //     [1, 2); a 128-row table (twmc_f64tab.h) indexed by m's top 7 fraction
-//     bits gives c_j and L_j with ln m = L_j + ln(1 + r), r = m c_j - 1
+//     bits gives 2 c_j and L_j with ln m = L_j + ln(1 + r), r = m c_j - 1
 //     exact and |r| <= 2^-7, ln(1 + r) = r + r^2 pq(r), plus e ln 2 with ln 2
 //     split so that e ln2_hi is exact;
 //   ps, pc (degree 2), pq (degree 6): Chebyshev fits over those ranges
@@ -82,6 +82,16 @@ TWMC_HD inline double from_bits(uint64_t u) {
   return x;
 }
 
+// The significand of x > 0 scaled into [1/2, 1), exactly (one v_frexp_mant_f64).
+TWMC_HD inline double frexp_mant(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_frexp_mant(x);
+#else
+  int e;
+  return frexp(x, &e);
+#endif
+}
+
 // Seed for the square root's Newton steps: the hardware's approximate
 // 1/sqrt(x) on the GPU.  On the host the correctly rounded value cut to 21
 // significant bits, a seed no better than the hardware's, so the accuracy test
@@ -140,7 +150,7 @@ TWMC_HD inline void pin(Series& k) {
 #endif
 }
 
-// The tables (TWMC_F64_TABLES, 768 doubles): the ln rows (c_j, L_j) from
+// The tables (TWMC_F64_TABLES, 768 doubles): the ln rows (2 c_j, L_j) from
 // kTabLog, the sin / cos rows (cos theta_j, sin theta_j) from kTabSinCos.  On
 // the host the functions read them from kTables; device code hands them a copy
 // in device memory.
@@ -189,7 +199,7 @@ TWMC_HD inline void sincos_turn32(const uint32_t* b, double* s, double* c, const
   }
 }
 
-// ln((a[i] + 1/2) 2^-32), in (-22.9, 0), i < N.  A row (c, L) of the ln
+// ln((a[i] + 1/2) 2^-32), in (-22.9, 0), i < N.  A row (2 c, L) of the ln
 // table: r = m c - 1 and ln m = L + ln(1 + r), the exponent raised by one for
 // m > sqrt 2, which L absorbs as -ln 2.
 template <int N>
@@ -203,14 +213,14 @@ TWMC_HD inline void log_u32(const uint32_t* a, double* out, const Series& k = kS
   for (int i = 0; i < N; ++i) {
     // a + 1/2 = m 2^(E - 1023) exactly, m in [1, 2) with at most 33 significant
     // bits; row j from its top 7 fraction bits; u = m 2^(E - 1055)
-    const uint64_t vb = bits(double(a[i]) + 0.5);
-    const uint32_t hi = uint32_t(vb >> 32);
+    const double v = double(a[i]) + 0.5;
+    const uint32_t hi = uint32_t(bits(v) >> 32);
     const uint32_t j = (hi >> 13) & 127u;
     // E - 1055, plus one for rows j >= 53: adding 75 to j carries into E
     e[i] = int((hi + (75u << 13)) >> 20) - 1055;
-    const double m = from_bits((uint64_t((hi & 0xfffffu) | 0x3ff00000u) << 32) | (vb & 0xffffffffull));
     const Row row = rows[j];
-    r[i] = fma(m, row.a, -1.0);  // exact: |r| <= 2^-7, m c has at most 52 bits
+    // m / 2 (the significand scaled into [1/2, 1)) times the row's 2 c
+    r[i] = fma(frexp_mant(v), row.a, -1.0);  // exact: |r| <= 2^-7, m c has at most 52 bits
     lt[i] = row.b;
     p[i] = k.l_top;
   }

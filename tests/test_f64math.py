@@ -35,7 +35,7 @@ def _table_rows(name):
 
 def test_tables_match_their_definition():
     """The committed tables against their definition at 40 digits: ln rows
-    (c_j with at most 19 significant bits, |m c_j - 1| <= 2^-7 over the row's
+    (stored as 2 c_j; c_j with at most 19 significant bits, |m c_j - 1| <= 2^-7 over the row's
     interval, L_j = -ln c_j - [j >= 53] ln 2 within 0.01 of its ulp, exact
     rows c = 1, 1/2 at the ends), and sin / cos rows the doubles nearest
     cos, sin (2 pi j / 256)."""
@@ -48,7 +48,7 @@ def test_tables_match_their_definition():
     sc_rows = _table_rows("TWMC_F64_SINCOS_ROWS")
     assert len(log_rows) == 256 and len(sc_rows) == 512
     for j in range(128):
-        c, L = log_rows[2 * j], log_rows[2 * j + 1]
+        c, L = log_rows[2 * j] / 2, log_rows[2 * j + 1]  # stored as 2 c_j
         m_, _ = math.frexp(c)
         assert m_ * 2 ** 19 == int(m_ * 2 ** 19), j
         lo, hi = 1 + mp.mpf(j) / 128, 1 + mp.mpf(j + 1) / 128
@@ -58,7 +58,7 @@ def test_tables_match_their_definition():
             assert want == 0, j
         else:
             assert abs(mp.mpf(L) - want) <= 0.01 * math.ulp(abs(L)), j
-    assert log_rows[:2] == [1.0, 0.0] and log_rows[-2:] == [0.5, 0.0]
+    assert log_rows[:2] == [2.0, 0.0] and log_rows[-2:] == [1.0, 0.0]
     for j in range(256):
         t = mp.mpf(j) / 128  # the angle in units of pi
         assert sc_rows[2 * j] == float(mp.cospi(t)) and sc_rows[2 * j + 1] == float(mp.sinpi(t)), j
