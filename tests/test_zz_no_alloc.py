@@ -27,6 +27,7 @@ def square(nx, ny, steps):
 
 CASES = [("stencil64", 32, square(64, 64, 30), 96, 40),
          ("generic", 64, square(12, 10, 30), 20, 7),
+         ("generic", 64, square(64, 64, 30), 300, 40),
          ("generic", 32, square(17, 5, 30), 33, 9),
          ("band", 32, dict(c3_spec(cells=128, t_end=1.0, frame_steps=30)), 100, 33),
          ("mfma", 32, dict(c3_spec(cells=128, t_end=1.0, frame_steps=30)), 70, 33)]
@@ -37,7 +38,8 @@ def allocating(eng, ids, b0, step0, steps, dt):
                    t_start=step0 * dt, t_end=(step0 + steps) * dt, final=True)
 
 
-@pytest.mark.parametrize("kernel,prec,spec,b1,b2", CASES, ids=[f"{c[0]}-fp{c[1]}" for c in CASES])
+@pytest.mark.parametrize("kernel,prec,spec,b1,b2", CASES,
+                         ids=[f"{c[0]}-fp{c[1]}-{c[2]['nx']}x{c[2]['ny']}" for c in CASES])
 def test_noalloc_equals_the_allocating_entry_point(kernel, prec, spec, b1, b2):
     steps, dt = 30, spec["dt"]
     n = spec["nx"] * spec["ny"]
