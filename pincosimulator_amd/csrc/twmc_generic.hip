@@ -61,7 +61,8 @@ __device__ __forceinline__ double warp_sum(double v) {
 
 // NN: a step launch with the hopping in its nearest-neighbour form and no
 // frame capture (the capture runs as its own launch): the kernel then needs
-// one LDS copy of the state and none of the FFT's registers.
+// one LDS copy of the state (two with BLOCK 1024 and P2, see kDB) and none of
+// the FFT's registers.
 // UNIF (with NN): the site tables are uniform (StepArgs::site_uniform): their
 // first entries are read once, into registers, instead of per site and step.
 // P2 (with NN): ny a power of two >= 2 and nx = 1 or a power of two >= 2, so
