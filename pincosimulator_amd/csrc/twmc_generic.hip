@@ -73,7 +73,6 @@ __device__ __forceinline__ double warp_sum(double v) {
 template <typename T, int PPT, bool NN = false, bool UNIF = false, bool P2 = false, int BLOCK = kBlock>
 __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   static_assert(!UNIF || NN, "site-uniform launches are nearest-neighbour launches");
-  static_assert(!P2 || UNIF, "the mask form is instantiated for site-uniform launches");
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = g.n;
   double* red = reinterpret_cast<double*>(smem);  // 8 doubles: per-wave partial sums
@@ -428,7 +427,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
           if constexpr (P2) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-              if constexpr (sizeof(T) == 8) {
+              if constexpr (sizeof(T) == 8 && UNIF) {
                 // three-operand FMAs: the new value can take the old one's
                 // registers (the accumulator form would need a copy)
                 const cpx<T> z = beta[2 * j + h];
@@ -664,8 +663,8 @@ const void* generic_nn_ptr_u(int ppt, int block) {
                     : (const void*)generic_step_kernel<T, 32, true, UNIF, P2>;
 }
 
-// power-of-two lattices (the mask form of the neighbour indices) are
-// instantiated for site-uniform parameters only
+// power-of-two lattices: the mask form of the neighbour indices and the
+// split LDS layout
 bool pow2_lattice(int nx, int ny) {
   auto p2 = [](int v) { return v >= 2 && (v & (v - 1)) == 0; };
   return p2(ny) && (nx == 1 || p2(nx));
@@ -675,11 +674,11 @@ const void* generic_nn_ptr(int precision, int ppt, bool unif, bool p2, int block
   if (precision == 64) {
     if (unif)
       return p2 ? generic_nn_ptr_u<double, true, true>(ppt, block) : generic_nn_ptr_u<double, true, false>(ppt, block);
-    return generic_nn_ptr_u<double, false, false>(ppt, block);
+    return p2 ? generic_nn_ptr_u<double, false, true>(ppt, block) : generic_nn_ptr_u<double, false, false>(ppt, block);
   }
   if (unif)
     return p2 ? generic_nn_ptr_u<float, true, true>(ppt, block) : generic_nn_ptr_u<float, true, false>(ppt, block);
-  return generic_nn_ptr_u<float, false, false>(ppt, block);
+  return p2 ? generic_nn_ptr_u<float, false, true>(ppt, block) : generic_nn_ptr_u<float, false, false>(ppt, block);
 }
 
 // threads per workgroup of a nearest-neighbour step launch
