@@ -61,16 +61,19 @@ __device__ __forceinline__ double warp_sum(double v) {
 
 // NN: a step launch with the hopping in its nearest-neighbour form and no
 // frame capture (the capture runs as its own launch): the kernel then needs
-// one LDS copy of the state (two with BLOCK 1024 and P2, see kDB) and none of
-// the FFT's registers.
+// one LDS copy of the state (two with BLOCK 1024 and a split layout, see kDB)
+// and none of the FFT's registers.
 // UNIF (with NN): the site tables are uniform (StepArgs::site_uniform): their
 // first entries are read once, into registers, instead of per site and step.
-// P2 (with NN): ny a power of two >= 2 and nx = 1 or a power of two >= 2, so
-// the neighbour indices are masks instead of a division and wrap tests.
+// LAY (with NN): the LDS layout of the staged state.  0: site order.  1 (ny
+// even): split -- even sites in the first half, odd sites in the second, a
+// pair's two sites in one row, neighbour indices by a division and wrap tests.
+// 2 (ny a power of two >= 2, nx = 1 or a power of two >= 2): split, the
+// neighbour indices as masks.
 // BLOCK: threads per workgroup (kBlock, or 1024 for the FP64 site-uniform
 // step launch of 2049 to 4096 sites: two pairs per thread fit 128 registers,
 // four waves per SIMD in the one workgroup per CU that the LDS state allows).
-template <typename T, int PPT, bool NN = false, bool UNIF = false, bool P2 = false, int BLOCK = kBlock>
+template <typename T, int PPT, bool NN = false, bool UNIF = false, int LAY = 0, int BLOCK = kBlock>
 __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   static_assert(!UNIF || NN, "site-uniform launches are nearest-neighbour launches");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -104,7 +107,8 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   // DB (the 1024-thread FP64 launch on a power-of-two lattice, one workgroup
   // per CU): two LDS copies of the state, each step reading one and writing
   // the new values into the other, so one barrier per step instead of two
-  constexpr bool kDB = NN && P2 && BLOCK == 1024;
+  constexpr bool kSplit = LAY != 0;  // split layout, pairs within rows
+  constexpr bool kDB = NN && kSplit && BLOCK == 1024;
   cpx<T>* const bufD = bufA + n;
   const double* ltab = kF64TabDev;
   if constexpr (NN && sizeof(T) == 8) {
@@ -117,6 +121,8 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   constexpr bool kStencil = sizeof(T) == 4 || NN;
   // x = floor(i / ny) as umulhi(i, ceil(2^32 / ny)): exact for i ny < 2^32
   const uint32_t ymagic = g.ny > 1 ? (uint32_t)((0x100000000ull + (uint64_t)g.ny - 1) / (uint64_t)g.ny) : 0u;
+  // pp / (ny / 2) as umulhi(pp, pmagic) for the split layout's row of a pair
+  const uint32_t pmagic = g.ny > 1 ? (uint32_t)((0x100000000ull + (uint64_t)(g.ny >> 1) - 1) / (uint64_t)(g.ny >> 1)) : 0u;
   const T jval = T(-0.5 * g.kscale);  // J
   const T dt = T(g.dt), dt4 = T(g.dt4), kscale = T(g.kscale);
   const T djv = T(-0.5 * g.kscale * g.dt);  // dt J
@@ -163,7 +169,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
       __syncthreads();
     }
     for (int s = 0; s < iters; ++s) {
-      // stage beta in LDS.  P2: even sites in the first half, odd sites in the
+      // stage beta in LDS.  Split layout: even sites in the first half, odd sites in the
       // second (site 2p at p, 2p + 1 at n/2 + p), so a wave's stores and
       // neighbour reads step 16 bytes from lane to lane (no bank conflicts).
       // DB: the state of this step is already in nbuf, the new one goes to nxt
@@ -173,7 +179,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
       for (int j = 0; j < PPT && !kDB; ++j) {
         int p = threadIdx.x + BLOCK * j;
         if (p < npairs) {
-          if constexpr (P2) {
+          if constexpr (kSplit) {
             bufA[p] = beta[2 * j];
             bufA[npairs + p] = beta[2 * j + 1];
           } else {
@@ -309,29 +315,38 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
           u32x4 x = philox4x32_10_wave(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
           T xr[2], xi[2];
           box_muller_pair<T>(x, xr, xi, bmk, ltab);
-          // P2: the pair's sites 2p, 2p + 1 share a row (ny even) and are each
+          // split layout: the pair's sites 2p, 2p + 1 share a row (ny even) and are each
           // other's y neighbours; their other y neighbours and their x
           // neighbours are read from the split LDS layout.
           // Both sites' new values are formed before either is stored, so beta
-          // holds the pair's state before this step throughout (P2).
+          // holds the pair's state before this step throughout.
           cpx<T> znew[2], dz[2];
           T c1[2];
           int nbl = 0, nbr = 0, nbu = 0, nbd = 0;
-          if constexpr (P2) {
+          if constexpr (kSplit) {
             // in pairs (yh pairs per row): the left neighbour of 2p is the odd
             // site of pair rb + q - 1, the right neighbour of 2p + 1 the even
             // site of pair rb + q + 1 (q the pair's place in its row, mod yh);
             // the x neighbours are the pairs p +- yh (mod n/2), both sites
-            const int pp = p + opq, yh = g.ny >> 1, q = pp & (yh - 1), rb = pp - q, hm = npairs - 1;
-            nbl = npairs + rb + ((q - 1) & (yh - 1));
-            nbr = rb + ((q + 1) & (yh - 1));
-            nbu = (pp + yh) & hm;
-            nbd = (pp - yh) & hm;
+            const int pp = p + opq, yh = g.ny >> 1;
+            if constexpr (LAY == 2) {
+              const int q = pp & (yh - 1), rb = pp - q, hm = npairs - 1;
+              nbl = npairs + rb + ((q - 1) & (yh - 1));
+              nbr = rb + ((q + 1) & (yh - 1));
+              nbu = (pp + yh) & hm;
+              nbd = (pp - yh) & hm;
+            } else {
+              const int q = pp - (int)__umulhi((uint32_t)pp, pmagic) * yh;
+              nbl = npairs + pp - 1 + (q == 0 ? yh : 0);
+              nbr = pp + 1 - (q == yh - 1 ? yh : 0);
+              nbu = pp + yh >= npairs ? pp + yh - npairs : pp + yh;
+              nbd = pp < yh ? pp - yh + npairs : pp - yh;
+            }
           }
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             int i = 2 * p + h + opq;
-            if (P2 || i < n) {  // P2: n even
+            if (kSplit || i < n) {  // split: n even
               cpx<T> z = beta[2 * j + h];
               cpx<T> hh = cur[i];
               T sr = T(0), si = T(0);
@@ -339,7 +354,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
                 if constexpr (kStencil) {
                   // h = -i J S, S the sum over the periodic nearest neighbours
                   // (a dimension of length 1 has none): IDFT(K DFT beta)/N exactly
-                  if constexpr (P2) {
+                  if constexpr (kSplit) {
                     // y + 1 and y - 1 neighbours, then x + 1 and x - 1
                     const cpx<T> a = h == 0 ? beta[2 * j + 1] : nbuf[nbr];
                     const cpx<T> c = h == 0 ? nbuf[nbl] : beta[2 * j];
@@ -389,7 +404,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
                   T di = fma(dt4, xi[h], fd.im);
                   di = fma(-djv, sr, di);
                   di = fma(ci, z.re, di);
-                  if constexpr (P2) {
+                  if constexpr (kSplit) {
                     // applied after the pair's other site has read this one
                     c1[h] = cr1;
                     dz[h] = mk(dr, di);
@@ -418,13 +433,13 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
                 cpx<T> a = mk(sdr.re * dt, sdr.im * dt);
                 z = mk(z.re + a.re + dt4 * xr[h], z.im + a.im + dt4 * xi[h]);
               }
-              if constexpr (P2)
+              if constexpr (kSplit)
                 znew[h] = z;
               else
                 beta[2 * j + h] = z;
             }
           }
-          if constexpr (P2) {
+          if constexpr (kSplit) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
               if constexpr (sizeof(T) == 8 && UNIF) {
@@ -653,32 +668,35 @@ hipError_t launch_generic_p(const StepArgs& a, int grid, int block, size_t lds, 
   }
 }
 
-template <typename T, bool UNIF, bool P2>
-const void* generic_nn_ptr_u(int ppt, int block) {
+template <typename T, bool UNIF, int LAY>
+const void* generic_nn_ptr_l(int ppt, int block) {
   if constexpr (sizeof(T) == 8 && UNIF)
-    if (block == 1024) return (const void*)generic_step_kernel<T, 2, true, UNIF, P2, 1024>;
-  return ppt == 1   ? (const void*)generic_step_kernel<T, 1, true, UNIF, P2>
-         : ppt == 2 ? (const void*)generic_step_kernel<T, 2, true, UNIF, P2>
-         : ppt == 8 ? (const void*)generic_step_kernel<T, 8, true, UNIF, P2>
-                    : (const void*)generic_step_kernel<T, 32, true, UNIF, P2>;
+    if (block == 1024) return (const void*)generic_step_kernel<T, 2, true, UNIF, LAY, 1024>;
+  return ppt == 1   ? (const void*)generic_step_kernel<T, 1, true, UNIF, LAY>
+         : ppt == 2 ? (const void*)generic_step_kernel<T, 2, true, UNIF, LAY>
+         : ppt == 8 ? (const void*)generic_step_kernel<T, 8, true, UNIF, LAY>
+                    : (const void*)generic_step_kernel<T, 32, true, UNIF, LAY>;
 }
 
-// power-of-two lattices: the mask form of the neighbour indices and the
-// split LDS layout
-bool pow2_lattice(int nx, int ny) {
+template <typename T, bool UNIF>
+const void* generic_nn_ptr_u(int ppt, int lay, int block) {
+  return lay == 2   ? generic_nn_ptr_l<T, UNIF, 2>(ppt, block)
+         : lay == 1 ? generic_nn_ptr_l<T, UNIF, 1>(ppt, block)
+                    : generic_nn_ptr_l<T, UNIF, 0>(ppt, block);
+}
+
+// the LDS layout of a nearest-neighbour step launch (LAY): split when a pair's
+// two sites share a row, with mask indices on power-of-two lattices
+int nn_layout(int nx, int ny) {
   auto p2 = [](int v) { return v >= 2 && (v & (v - 1)) == 0; };
-  return p2(ny) && (nx == 1 || p2(nx));
+  if (p2(ny) && (nx == 1 || p2(nx))) return 2;
+  return (ny >= 2 && ny % 2 == 0) ? 1 : 0;
 }
 
-const void* generic_nn_ptr(int precision, int ppt, bool unif, bool p2, int block) {
-  if (precision == 64) {
-    if (unif)
-      return p2 ? generic_nn_ptr_u<double, true, true>(ppt, block) : generic_nn_ptr_u<double, true, false>(ppt, block);
-    return p2 ? generic_nn_ptr_u<double, false, true>(ppt, block) : generic_nn_ptr_u<double, false, false>(ppt, block);
-  }
-  if (unif)
-    return p2 ? generic_nn_ptr_u<float, true, true>(ppt, block) : generic_nn_ptr_u<float, true, false>(ppt, block);
-  return p2 ? generic_nn_ptr_u<float, false, true>(ppt, block) : generic_nn_ptr_u<float, false, false>(ppt, block);
+const void* generic_nn_ptr(int precision, int ppt, bool unif, int lay, int block) {
+  if (precision == 64)
+    return unif ? generic_nn_ptr_u<double, true>(ppt, lay, block) : generic_nn_ptr_u<double, false>(ppt, lay, block);
+  return unif ? generic_nn_ptr_u<float, true>(ppt, lay, block) : generic_nn_ptr_u<float, false>(ppt, lay, block);
 }
 
 // threads per workgroup of a nearest-neighbour step launch
@@ -693,7 +711,7 @@ int generic_nn_block(int precision, int n, bool unif) {
 // Box-Muller tables
 size_t generic_nn_lds_bytes(int nx, int ny, int precision, bool site_uniform) {
   const int n = nx * ny;
-  const bool db = generic_nn_block(precision, n, site_uniform) == 1024 && pow2_lattice(nx, ny);
+  const bool db = generic_nn_block(precision, n, site_uniform) == 1024 && nn_layout(nx, ny) != 0;
   return 64 + (size_t)n * (precision == 64 ? 16 : 8) * (db ? 2 : 1) +
          (precision == 64 ? sizeof(double) * f64::kTabSize : 0);
 }
@@ -702,7 +720,7 @@ hipError_t generic_nn_occupancy(int precision, int nx, int ny, size_t lds, bool 
                                 int* blocks_per_cu) {
   const int block = generic_nn_block(precision, nx * ny, site_uniform);
   const int ppt = generic_pairs_per_thread(nx * ny, block);
-  const void* f = generic_nn_ptr(precision, ppt, site_uniform, pow2_lattice(nx, ny), block);
+  const void* f = generic_nn_ptr(precision, ppt, site_uniform, nn_layout(nx, ny), block);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -713,7 +731,7 @@ hipError_t generic_nn_occupancy(int precision, int nx, int ny, size_t lds, bool 
 hipError_t launch_generic_nn(const StepArgs& a, int precision, int grid, size_t lds, hipStream_t s) {
   const int block = generic_nn_block(precision, a.n, a.site_uniform != 0);
   const int ppt = generic_pairs_per_thread(a.n, block);
-  const void* f = generic_nn_ptr(precision, ppt, a.site_uniform != 0, pow2_lattice(a.nx, a.ny), block);
+  const void* f = generic_nn_ptr(precision, ppt, a.site_uniform != 0, nn_layout(a.nx, a.ny), block);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;

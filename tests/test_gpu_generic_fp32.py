@@ -37,10 +37,11 @@ def test_fp32_nearest_neighbour_matches_fft_oracle(nx, ny):
     assert rel_err(res.sum_nk, nk_ref.reshape(res.sum_nk.shape)) < 2e-5
 
 
-# power-of-two lattices (ny >= 2, nx = 1 or >= 2) take the step launch whose
-# neighbour indices are masks; (8, 6) and (32, 1) the division form; 2049 to
-# 4096 sites the 1024-thread launch (double-buffered state on power-of-two
-# lattices: (4, 1024), (128, 32), (32, 128); (48, 48) single-buffered)
+# power-of-two lattices (ny >= 2, nx = 1 or >= 2) take the split-layout step
+# launch whose neighbour indices are masks; other even row lengths ((13, 14),
+# (8, 6), (48, 48)) the split layout with the division form; odd row lengths
+# and (32, 1) the site-order layout; 2049 to 4096 sites the 1024-thread launch,
+# double-buffered in the split layouts: (4, 1024), (128, 32), (32, 128), (48, 48)
 @pytest.mark.parametrize("nx,ny", [(1, 7), (5, 7), (1, 127), (7, 11), (13, 14),
                                    (1, 2), (1, 32), (2, 8), (4, 2), (16, 4), (8, 6), (32, 1),
                                    (4, 1024), (128, 32), (32, 128), (48, 48)])
@@ -59,10 +60,12 @@ def test_fp64_awkward_and_power_of_two_shapes_match_oracle(nx, ny):
 
 
 # site-varying parameters (a site-dependent detuning, per-trajectory U and F
-# disorder) on power-of-two lattices: the mask-index, split-layout step launch
-# that reads the parameter tables per site
-@pytest.mark.parametrize("nx,ny,prec", [(16, 16, 32), (16, 16, 64), (1, 64, 32), (64, 64, 64), (8, 32, 32)])
-def test_site_varying_parameters_power_of_two_shapes(nx, ny, prec):
+# disorder): the split-layout step launch that reads the parameter tables per
+# site, with mask indices on power-of-two lattices and the division form on
+# other even row lengths ((6, 10), (12, 14), (40, 48))
+@pytest.mark.parametrize("nx,ny,prec", [(16, 16, 32), (16, 16, 64), (1, 64, 32), (64, 64, 64), (8, 32, 32),
+                                        (6, 10, 64), (12, 14, 32), (40, 48, 32)])
+def test_site_varying_parameters_split_layout(nx, ny, prec):
     n = nx * ny
     spec = dict(nx=nx, ny=ny, dt=0.005, gamma=1.0, J=1.1, omega=0.4 + 0.3 * np.cos(0.7 * np.arange(n)),
                 U={"base": 0.1, "noise": ("gaussian", 0.02 + 0.01j)}, F={"base": 1.2, "noise": ("uniform", 0.3)},

@@ -104,8 +104,8 @@ def test_square_fp64(nx, ny):
 def test_fp64_more_trajectories_than_workgroups(nx, ny, n_traj):
     """The persistent step launch: more trajectories than workgroups, so
     workgroups step a second and third trajectory after the first (the
-    64x64 launch, 1024 threads and two LDS copies of the state per workgroup,
-    one workgroup per CU; 8x8, 256 threads; 48x48, 1024 threads, one copy).
+    64x64 and 48x48 launches, 1024 threads and two LDS copies of the state per
+    workgroup, one workgroup per CU; 8x8, 256 threads).
     Trajectories from the first, second and last round of the grid against the
     oracle."""
     spec = dict(nx=nx, ny=ny, dt=0.002, gamma=1.0, J=1.0, U=0.05, omega=1.0, F=2.0, beta_init=0.0,
