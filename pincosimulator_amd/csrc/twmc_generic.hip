@@ -76,6 +76,7 @@ __device__ __forceinline__ double warp_sum(double v) {
 template <typename T, int PPT, bool NN = false, bool UNIF = false, int LAY = 0, int BLOCK = kBlock>
 __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   static_assert(!UNIF || NN, "site-uniform launches are nearest-neighbour launches");
+  static_assert(LAY == 0 || NN, "the split layouts are nearest-neighbour launches");
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = g.n;
   double* red = reinterpret_cast<double*>(smem);  // 8 doubles: per-wave partial sums
@@ -348,7 +349,8 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
             int i = 2 * p + h + opq;
             if (kSplit || i < n) {  // split: n even
               cpx<T> z = beta[2 * j + h];
-              cpx<T> hh = cur[i];
+              cpx<T> hh = mk(T(0), T(0));  // -i J S, or the FFT form's product
+              if constexpr (!NN) hh = cur[i];
               T sr = T(0), si = T(0);
               if (NN || g.hopping == 0) {
                 if constexpr (kStencil) {
