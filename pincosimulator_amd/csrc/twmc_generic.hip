@@ -110,6 +110,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   // the new values into the other, so one barrier per step instead of two
   constexpr bool kSplit = LAY != 0;  // split layout, pairs within rows
   constexpr bool kDB = NN && kSplit && BLOCK == 1024;
+  constexpr bool kBatch = kSplit && (UNIF || sizeof(T) == 4);  // neighbour reads batched
   cpx<T>* const bufD = bufA + n;
   const double* ltab = kF64TabDev;
   if constexpr (NN && sizeof(T) == 8) {
@@ -312,10 +313,6 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
       for (int j = 0; j < PPT; ++j) {
         int p = threadIdx.x + BLOCK * j;
         if (p < npairs) {
-          // the trajectory (one per workgroup) and step are wave-uniform
-          u32x4 x = philox4x32_10_wave(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
-          T xr[2], xi[2];
-          box_muller_pair<T>(x, xr, xi, bmk, ltab);
           // split layout: the pair's sites 2p, 2p + 1 share a row (ny even) and are each
           // other's y neighbours; their other y neighbours and their x
           // neighbours are read from the split LDS layout.
@@ -344,6 +341,44 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
               nbd = pp < yh ? pp - yh + npairs : pp - yh;
             }
           }
+          // split layout (kBatch): the pair's six neighbour reads issued
+          // together (they need only the addresses), then the Philox rounds,
+          // and only then their use -- one LDS round trip, under the integer
+          // work (the scheduling barrier keeps the compiler from pulling the
+          // sums, and so the waits, up to each read).  Not for FP64 with
+          // site-varying parameters, where the held values would cost a wave
+          // per SIMD; those read at the use below.
+          T nsr[2] = {T(0), T(0)}, nsi[2] = {T(0), T(0)};
+          cpx<T> nl, nr, nu[2], nd[2];
+          if constexpr (kBatch) {
+            nl = nbuf[nbl];
+            nr = nbuf[nbr];
+            if (g.nx > 1) {
+              nu[0] = nbuf[nbu];
+              nd[0] = nbuf[nbd];
+              nu[1] = nbuf[npairs + nbu];
+              nd[1] = nbuf[npairs + nbd];
+            }
+          }
+          // the trajectory (one per workgroup) and step are wave-uniform
+          u32x4 x = philox4x32_10_wave(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
+          if constexpr (kBatch) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              // y + 1 and y - 1 neighbours, then x + 1 and x - 1
+              const cpx<T> a = h == 0 ? beta[2 * j + 1] : nr;
+              const cpx<T> c = h == 0 ? nl : beta[2 * j];
+              nsr[h] = a.re + c.re;
+              nsi[h] = a.im + c.im;
+              if (g.nx > 1) {
+                nsr[h] += nu[h].re + nd[h].re;
+                nsi[h] += nu[h].im + nd[h].im;
+              }
+            }
+          }
+          T xr[2], xi[2];
+          box_muller_pair<T>(x, xr, xi, bmk, ltab);
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             int i = 2 * p + h + opq;
@@ -356,7 +391,10 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
                 if constexpr (kStencil) {
                   // h = -i J S, S the sum over the periodic nearest neighbours
                   // (a dimension of length 1 has none): IDFT(K DFT beta)/N exactly
-                  if constexpr (kSplit) {
+                  if constexpr (kBatch) {
+                    sr = nsr[h];
+                    si = nsi[h];
+                  } else if constexpr (kSplit) {
                     // y + 1 and y - 1 neighbours, then x + 1 and x - 1
                     const cpx<T> a = h == 0 ? beta[2 * j + 1] : nbuf[nbr];
                     const cpx<T> c = h == 0 ? nbuf[nbl] : beta[2 * j];
