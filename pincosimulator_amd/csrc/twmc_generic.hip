@@ -70,9 +70,10 @@ __device__ __forceinline__ double warp_sum(double v) {
 // pair's two sites in one row, neighbour indices by a division and wrap tests.
 // 2 (ny a power of two >= 2, nx = 1 or a power of two >= 2): split, the
 // neighbour indices as masks.
-// BLOCK: threads per workgroup (kBlock, or 1024 for the FP64 site-uniform
-// step launch of 2049 to 4096 sites: two pairs per thread fit 128 registers,
-// four waves per SIMD in the one workgroup per CU that the LDS state allows).
+// BLOCK: threads per workgroup (kBlock; 64 or 128 for the step launch of a
+// lattice of up to 128 or 256 sites; 1024 for the FP64 site-uniform step
+// launch of 2049 to 4096 sites: two pairs per thread fit 128 registers, four
+// waves per SIMD in the one workgroup per CU that the LDS state allows).
 template <typename T, int PPT, bool NN = false, bool UNIF = false, int LAY = 0, int BLOCK = kBlock>
 __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   static_assert(!UNIF || NN, "site-uniform launches are nearest-neighbour launches");
@@ -712,6 +713,8 @@ template <typename T, bool UNIF, int LAY>
 const void* generic_nn_ptr_l(int ppt, int block) {
   if constexpr (sizeof(T) == 8 && UNIF)
     if (block == 1024) return (const void*)generic_step_kernel<T, 2, true, UNIF, LAY, 1024>;
+  if (block == 64) return (const void*)generic_step_kernel<T, 1, true, UNIF, LAY, 64>;
+  if (block == 128) return (const void*)generic_step_kernel<T, 1, true, UNIF, LAY, 128>;
   return ppt == 1   ? (const void*)generic_step_kernel<T, 1, true, UNIF, LAY>
          : ppt == 2 ? (const void*)generic_step_kernel<T, 2, true, UNIF, LAY>
          : ppt == 8 ? (const void*)generic_step_kernel<T, 8, true, UNIF, LAY>
@@ -739,8 +742,13 @@ const void* generic_nn_ptr(int precision, int ppt, bool unif, int lay, int block
   return unif ? generic_nn_ptr_u<float, true>(ppt, lay, block) : generic_nn_ptr_u<float, false>(ppt, lay, block);
 }
 
-// threads per workgroup of a nearest-neighbour step launch
+// threads per workgroup of a nearest-neighbour step launch: one or two waves
+// for lattices of up to 128 or 256 sites (a pair per thread, every wave busy),
+// 1024 for FP64 site-uniform lattices of 2049 to 4096 sites, else kBlock
 int generic_nn_block(int precision, int n, bool unif) {
+  const int np = (n + 1) / 2;
+  if (np <= 64) return 64;
+  if (np <= 128) return 128;
   return (precision == 64 && unif && generic_pairs_per_thread(n, 1024) == 2) ? 1024 : kBlock;
 }
 
