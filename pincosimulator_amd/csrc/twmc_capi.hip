@@ -489,7 +489,8 @@ int step_geometry(Ctx* c, size_t B, Geometry& geo) {
       geo.lds_nn = generic_nn_lds_bytes(c->p.nx, c->p.ny, c->precision, site_uniform(c));
       HIPCHK(c, generic_nn_occupancy(c->precision, c->p.nx, c->p.ny, geo.lds_nn, site_uniform(c), &per_cu_nn));
       if (per_cu_nn < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");
-      geo.grid_nn = (int)std::min<size_t>(B, (size_t)per_cu_nn * cus);
+      const size_t tpb = (size_t)generic_nn_traj_per_block(c->n);
+      geo.grid_nn = (int)std::min<size_t>((B + tpb - 1) / tpb, (size_t)per_cu_nn * cus);
     }
   }
   if (per_cu < 1) return fail(c, TWMC_ERR_UNSUPPORTED, "step kernel does not fit on a CU");

@@ -74,14 +74,22 @@ __device__ __forceinline__ double warp_sum(double v) {
 // lattice of up to 128 or 256 sites; 1024 for the FP64 site-uniform step
 // launch of 2049 to 4096 sites: two pairs per thread fit 128 registers, four
 // waves per SIMD in the one workgroup per CU that the LDS state allows).
-template <typename T, int PPT, bool NN = false, bool UNIF = false, int LAY = 0, int BLOCK = kBlock>
+// TPB: trajectories per workgroup (64-thread step launches of lattices of up
+// to 64 sites: 2, or 4 for up to 32 sites), each with BLOCK / TPB lanes and
+// its own LDS copy of the state.
+template <typename T, int PPT, bool NN = false, bool UNIF = false, int LAY = 0, int BLOCK = kBlock, int TPB = 1>
 __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   static_assert(!UNIF || NN, "site-uniform launches are nearest-neighbour launches");
   static_assert(LAY == 0 || NN, "the split layouts are nearest-neighbour launches");
+  static_assert(TPB == 1 || (NN && PPT == 1 && BLOCK == 64), "packed trajectories: 64-thread step launches");
+  constexpr int kLanes = BLOCK / TPB;  // threads per trajectory
+  const int tg = TPB > 1 ? (int)threadIdx.x / kLanes : 0;  // this thread's trajectory in the workgroup
+  const int tl = TPB > 1 ? (int)threadIdx.x % kLanes : (int)threadIdx.x;
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = g.n;
   double* red = reinterpret_cast<double*>(smem);  // 8 doubles: per-wave partial sums
-  cpx<T>* bufA = reinterpret_cast<cpx<T>*>(smem + 64);
+  cpx<T>* const base0 = reinterpret_cast<cpx<T>*>(smem + 64);
+  cpx<T>* bufA = base0 + tg * n;
   cpx<T>* bufB = bufA + n;
   cpx<T>* twx = bufB + n;
   cpx<T>* twy = twx + g.nx;
@@ -115,7 +123,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   cpx<T>* const bufD = bufA + n;
   const double* ltab = kF64TabDev;
   if constexpr (NN && sizeof(T) == 8) {
-    double* lw = reinterpret_cast<double*>(bufA + (kDB ? 2 * n : n));
+    double* lw = reinterpret_cast<double*>(base0 + (kDB ? 2 * n : TPB * n));
     for (int i = threadIdx.x; i < f64::kTabSize; i += blockDim.x) lw[i] = kF64TabDev[i];
     ltab = lw;
     __syncthreads();
@@ -130,7 +138,11 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   const T dt = T(g.dt), dt4 = T(g.dt4), kscale = T(g.kscale);
   const T djv = T(-0.5 * g.kscale * g.dt);  // dt J
 
-  for (int b = blockIdx.x; b < g.n_traj; b += gridDim.x) {
+  for (int b0 = blockIdx.x * TPB; b0 < g.n_traj; b0 += gridDim.x * TPB) {
+    // a group past the batch's end steps the last trajectory again and stores
+    // nothing, so that every thread reaches every barrier
+    const bool live = b0 + tg < g.n_traj;
+    const int b = live ? b0 + tg : g.n_traj - 1;
     const uint64_t traj = g.traj_ids[b];
     cpx<T>* gbeta = reinterpret_cast<cpx<T>*>(g.beta) + (size_t)b * n;
     const size_t poff = (size_t)(g.param_stride ? b : 0) * n;
@@ -151,7 +163,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
     cpx<T> beta[2 * PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-      int p = threadIdx.x + BLOCK * j;
+      int p = tl + kLanes * j;
       if (p < npairs) {
         beta[2 * j] = gbeta[2 * p];
         if (2 * p + 1 < n) beta[2 * j + 1] = gbeta[2 * p + 1];
@@ -163,7 +175,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
     if constexpr (kDB) {
 #pragma unroll
       for (int j = 0; j < PPT; ++j) {
-        const int p = threadIdx.x + BLOCK * j;
+        const int p = tl + kLanes * j;
         if (p < npairs) {
           bufA[p] = beta[2 * j];
           bufA[npairs + p] = beta[2 * j + 1];
@@ -180,7 +192,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
       cpx<T>* const nxt = (kDB && (s & 1)) ? bufA : bufD;
 #pragma unroll
       for (int j = 0; j < PPT && !kDB; ++j) {
-        int p = threadIdx.x + BLOCK * j;
+        int p = tl + kLanes * j;
         if (p < npairs) {
           if constexpr (kSplit) {
             bufA[p] = beta[2 * j];
@@ -212,7 +224,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
         double m2 = 0.0, m4 = 0.0;
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
-          int p = threadIdx.x + BLOCK * j;
+          int p = tl + kLanes * j;
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             int i = 2 * p + h;
@@ -259,7 +271,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
           // the transform overwrote bufA: stage beta again for the neighbour sums
 #pragma unroll
           for (int j = 0; j < PPT; ++j) {
-            int p = threadIdx.x + BLOCK * j;
+            int p = tl + kLanes * j;
             if (p < npairs) {
               bufA[2 * p] = beta[2 * j];
               if (2 * p + 1 < n) bufA[2 * p + 1] = beta[2 * j + 1];
@@ -312,7 +324,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
       }
 #pragma unroll
       for (int j = 0; j < PPT; ++j) {
-        int p = threadIdx.x + BLOCK * j;
+        int p = tl + kLanes * j;
         if (p < npairs) {
           // split layout: the pair's sites 2p, 2p + 1 share a row (ny even) and are each
           // other's y neighbours; their other y neighbours and their x
@@ -361,8 +373,13 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
               nd[1] = nbuf[npairs + nbd];
             }
           }
-          // the trajectory (one per workgroup) and step are wave-uniform
-          u32x4 x = philox4x32_10_wave(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
+          // the trajectory (one per workgroup) and step are wave-uniform,
+          // unless the workgroup packs several trajectories
+          u32x4 x;
+          if constexpr (TPB > 1)
+            x = philox4x32_10(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
+          else
+            x = philox4x32_10_wave(make_counter(traj, TAG_WIENER, step, (uint32_t)p), g.key0, g.key1);
           if constexpr (kBatch) {
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -504,8 +521,8 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
 
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-      int p = threadIdx.x + BLOCK * j;
-      if (p < npairs) {
+      int p = tl + kLanes * j;
+      if (live && p < npairs) {
         gbeta[2 * p] = beta[2 * j];
         if (2 * p + 1 < n) gbeta[2 * p + 1] = beta[2 * j + 1];
       }
@@ -710,10 +727,13 @@ hipError_t launch_generic_p(const StepArgs& a, int grid, int block, size_t lds, 
 }
 
 template <typename T, bool UNIF, int LAY>
-const void* generic_nn_ptr_l(int ppt, int block) {
+const void* generic_nn_ptr_l(int ppt, int block, int tpb) {
   if constexpr (sizeof(T) == 8 && UNIF)
     if (block == 1024) return (const void*)generic_step_kernel<T, 2, true, UNIF, LAY, 1024>;
-  if (block == 64) return (const void*)generic_step_kernel<T, 1, true, UNIF, LAY, 64>;
+  if (block == 64)
+    return tpb == 4   ? (const void*)generic_step_kernel<T, 1, true, UNIF, LAY, 64, 4>
+           : tpb == 2 ? (const void*)generic_step_kernel<T, 1, true, UNIF, LAY, 64, 2>
+                      : (const void*)generic_step_kernel<T, 1, true, UNIF, LAY, 64>;
   if (block == 128) return (const void*)generic_step_kernel<T, 1, true, UNIF, LAY, 128>;
   return ppt == 1   ? (const void*)generic_step_kernel<T, 1, true, UNIF, LAY>
          : ppt == 2 ? (const void*)generic_step_kernel<T, 2, true, UNIF, LAY>
@@ -722,10 +742,10 @@ const void* generic_nn_ptr_l(int ppt, int block) {
 }
 
 template <typename T, bool UNIF>
-const void* generic_nn_ptr_u(int ppt, int lay, int block) {
-  return lay == 2   ? generic_nn_ptr_l<T, UNIF, 2>(ppt, block)
-         : lay == 1 ? generic_nn_ptr_l<T, UNIF, 1>(ppt, block)
-                    : generic_nn_ptr_l<T, UNIF, 0>(ppt, block);
+const void* generic_nn_ptr_u(int ppt, int lay, int block, int tpb) {
+  return lay == 2   ? generic_nn_ptr_l<T, UNIF, 2>(ppt, block, tpb)
+         : lay == 1 ? generic_nn_ptr_l<T, UNIF, 1>(ppt, block, tpb)
+                    : generic_nn_ptr_l<T, UNIF, 0>(ppt, block, tpb);
 }
 
 // the LDS layout of a nearest-neighbour step launch (LAY): split when a pair's
@@ -736,10 +756,12 @@ int nn_layout(int nx, int ny) {
   return (ny >= 2 && ny % 2 == 0) ? 1 : 0;
 }
 
-const void* generic_nn_ptr(int precision, int ppt, bool unif, int lay, int block) {
+const void* generic_nn_ptr(int precision, int ppt, bool unif, int lay, int block, int tpb) {
   if (precision == 64)
-    return unif ? generic_nn_ptr_u<double, true>(ppt, lay, block) : generic_nn_ptr_u<double, false>(ppt, lay, block);
-  return unif ? generic_nn_ptr_u<float, true>(ppt, lay, block) : generic_nn_ptr_u<float, false>(ppt, lay, block);
+    return unif ? generic_nn_ptr_u<double, true>(ppt, lay, block, tpb)
+                : generic_nn_ptr_u<double, false>(ppt, lay, block, tpb);
+  return unif ? generic_nn_ptr_u<float, true>(ppt, lay, block, tpb)
+              : generic_nn_ptr_u<float, false>(ppt, lay, block, tpb);
 }
 
 // threads per workgroup of a nearest-neighbour step launch: one or two waves
@@ -752,23 +774,31 @@ int generic_nn_block(int precision, int n, bool unif) {
   return (precision == 64 && unif && generic_pairs_per_thread(n, 1024) == 2) ? 1024 : kBlock;
 }
 
+// trajectories per workgroup of a nearest-neighbour step launch (TPB)
+int nn_tpb(int n) {
+  const int np = (n + 1) / 2;
+  return np <= 16 ? 4 : np <= 32 ? 2 : 1;
+}
+
 }  // namespace
 
 // LDS of a nearest-neighbour step launch: the per-wave sums, one copy of the
-// state (two for the double-buffered 1024-thread FP64 launch) and, FP64, the
-// Box-Muller tables
+// state per trajectory of the workgroup (two for the double-buffered
+// 1024-thread FP64 launch) and, FP64, the Box-Muller tables
 size_t generic_nn_lds_bytes(int nx, int ny, int precision, bool site_uniform) {
   const int n = nx * ny;
   const bool db = generic_nn_block(precision, n, site_uniform) == 1024 && nn_layout(nx, ny) != 0;
-  return 64 + (size_t)n * (precision == 64 ? 16 : 8) * (db ? 2 : 1) +
+  return 64 + (size_t)n * (precision == 64 ? 16 : 8) * (db ? 2 : nn_tpb(n)) +
          (precision == 64 ? sizeof(double) * f64::kTabSize : 0);
 }
+
+int generic_nn_traj_per_block(int n) { return nn_tpb(n); }
 
 hipError_t generic_nn_occupancy(int precision, int nx, int ny, size_t lds, bool site_uniform,
                                 int* blocks_per_cu) {
   const int block = generic_nn_block(precision, nx * ny, site_uniform);
   const int ppt = generic_pairs_per_thread(nx * ny, block);
-  const void* f = generic_nn_ptr(precision, ppt, site_uniform, nn_layout(nx, ny), block);
+  const void* f = generic_nn_ptr(precision, ppt, site_uniform, nn_layout(nx, ny), block, nn_tpb(nx * ny));
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -779,7 +809,7 @@ hipError_t generic_nn_occupancy(int precision, int nx, int ny, size_t lds, bool 
 hipError_t launch_generic_nn(const StepArgs& a, int precision, int grid, size_t lds, hipStream_t s) {
   const int block = generic_nn_block(precision, a.n, a.site_uniform != 0);
   const int ppt = generic_pairs_per_thread(a.n, block);
-  const void* f = generic_nn_ptr(precision, ppt, a.site_uniform != 0, nn_layout(a.nx, a.ny), block);
+  const void* f = generic_nn_ptr(precision, ppt, a.site_uniform != 0, nn_layout(a.nx, a.ny), block, nn_tpb(a.n));
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;

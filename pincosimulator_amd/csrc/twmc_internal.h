@@ -178,6 +178,9 @@ int generic_pairs_per_thread(int n, int block);
 // the LDS kernel as a step launch with the hopping in its nearest-neighbour
 // form and no capture (FFT hopping): one LDS copy of the state
 size_t generic_nn_lds_bytes(int nx, int ny, int precision, bool site_uniform);
+// trajectories per workgroup of the nearest-neighbour step launch (1, 2 or 4):
+// its grid covers ceil(B / that) workgroups
+int generic_nn_traj_per_block(int n);
 hipError_t generic_nn_occupancy(int precision, int nx, int ny, size_t lds, bool site_uniform,
                                 int* blocks_per_cu);
 hipError_t launch_generic_nn(const StepArgs& a, int precision, int grid, size_t lds, hipStream_t s);

@@ -79,3 +79,33 @@ def test_site_varying_parameters_split_layout(nx, ny, prec):
     err = rel_err(res.traj, ref)
     print(f"  max relative deviation {err:.2e}")
     assert err < (1e-9 if prec == 64 else 2e-5)
+
+
+# lattices of up to 64 sites: 64-thread step launches that pack 2 (up to 64
+# sites) or 4 (up to 32 sites) trajectories per workgroup, each with its own
+# LDS state; odd batch sizes leave groups of the last workgroup past the end
+# of the batch (they must store nothing); per-trajectory disorder reads each
+# group's own parameter tables
+@pytest.mark.parametrize("nx,ny,prec,n_traj", [(1, 32, 64, 7), (4, 8, 32, 9), (8, 8, 64, 5), (1, 64, 32, 3),
+                                               (2, 12, 64, 11), (6, 10, 32, 13)])
+def test_packed_trajectories_small_lattices(nx, ny, prec, n_traj):
+    n = nx * ny
+    spec = dict(nx=nx, ny=ny, dt=0.005, gamma=1.0, J=0.9, omega=0.3 + 0.2 * np.sin(0.9 * np.arange(n)),
+                U={"base": 0.1, "noise": ("gaussian", 0.02)}, F={"base": 1.1, "noise": ("uniform", 0.2)},
+                beta_init=0.1, beta_sigma=0.5, frame_steps=40, seed=519, t_start=0.0, t_end=0.4)
+    ids = np.arange(n_traj) * 5 + 3
+    print(f"fp{prec} generic {nx}x{ny}, {n_traj} trajectories", flush=True)
+    eng = Engine(system(spec), precision=prec, kernel="generic")
+    res = eng.run(ids, traj=True)
+    eng.close()
+    ref = np.array([oracle.run_trajectory(dict(spec, lattice=oracle.LAT_FFT), int(t))[0] for t in ids])
+    err = rel_err(res.traj, ref)
+    print(f"  max relative deviation {err:.2e}")
+    assert err < (1e-9 if prec == 64 else 2e-5)
+    # site-uniform parameters too (the other instantiation)
+    spec_u = dict(spec, omega=0.3, U=0.1, F=1.1)
+    eng = Engine(system(spec_u), precision=prec, kernel="generic")
+    res = eng.run(ids, traj=True)
+    eng.close()
+    ref = np.array([oracle.run_trajectory(dict(spec_u, lattice=oracle.LAT_FFT), int(t))[0] for t in ids])
+    assert rel_err(res.traj, ref) < (1e-9 if prec == 64 else 2e-5)
