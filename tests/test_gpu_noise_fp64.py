@@ -7,7 +7,9 @@ so xi_i = (b' - b (1 + dt (-i omega_i - gamma/2))) / sqrt(gamma dt / 4) from
 two consecutive stored frames (one step apart).  The kernel draws xi from its
 own range-specialised FP64 Box-Muller (ln, sin and cos within 2 ulp); the
 oracle from glibc's log / sin / cos on the same Philox words.  Covered: the
-site-uniform launch (the parameters read once) and site-varying omega.
+site-uniform launch (the parameters read once) and site-varying omega on a
+small lattice, and the 64x64 site-uniform launch (1024 threads, two LDS
+copies of the state), there for the first step of every site.
 """
 import numpy as np
 import pytest
@@ -21,16 +23,18 @@ pytestmark = pytest.mark.gpu
 DT, GAMMA, SEED = 0.002, 1.0, 20261203
 
 
-def _spec(omega):
-    return dict(nx=8, ny=6, dt=DT, gamma=GAMMA, J=0.0, U=0.0, omega=omega, F=0.0, beta_init=0.3 + 0.1j,
+def _spec(omega, nx, ny):
+    return dict(nx=nx, ny=ny, dt=DT, gamma=GAMMA, J=0.0, U=0.0, omega=omega, F=0.0, beta_init=0.3 + 0.1j,
                 beta_sigma=0.0, frame_steps=1, seed=SEED, t_start=0.0, t_end=3 * DT)
 
 
-@pytest.mark.parametrize("site_uniform", [True, False])
-def test_fp64_increments_equal_oracle_normals(site_uniform):
-    n = 48
+# 8x6: the 64-thread launch with two trajectories per workgroup; 64x64 with
+# site-uniform parameters: the 1024-thread double-buffered launch
+@pytest.mark.parametrize("site_uniform,nx,ny", [(True, 8, 6), (False, 8, 6), (True, 64, 64)])
+def test_fp64_increments_equal_oracle_normals(site_uniform, nx, ny):
+    n = nx * ny
     omega = 0.7 if site_uniform else 0.5 + 0.25 * np.cos(np.arange(n))
-    spec = _spec(omega)
+    spec = _spec(omega, nx, ny)
     ids = np.array([0, 1, 7, 4097, (1 << 33) + 5], dtype=np.uint64)
     eng = Engine(system(spec), precision=64, kernel="generic")
     res = eng.run(ids, traj=True)
@@ -42,7 +46,7 @@ def test_fp64_increments_equal_oracle_normals(site_uniform):
     sig = np.sqrt(GAMMA * DT / 4.0)
     worst = 0.0
     for j, t in enumerate(ids):
-        for step in range(3):
+        for step in range(3 if n <= 64 else 1):
             xi = (traj[j, step + 1] - traj[j, step] * g) / sig
             ref = np.array([oracle.normal(SEED, int(t), oracle.TAG_WIENER, step, i) for i in range(n)])
             worst = max(worst, float(np.max(np.abs(xi - ref))))
