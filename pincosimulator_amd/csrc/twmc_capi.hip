@@ -274,14 +274,15 @@ int choose_kernel(Ctx* c) {
                 "stencil 64x64 kernel needs nx = ny = 64, FFT hopping, FP32 and site-uniform, disorder-free parameters");
   // AUTO: the 64x64 nearest-neighbour form first (the same linear map as the
   // reference's FFT hopping, about twice as fast as the register FFT), then
-  // the register FFT; other uniform FP32 lattices on the stacked LDS stencil;
-  // for matrix hopping the banded stencil when the coupling
-  // is banded (a dense product would mostly move zeros), else the MFMA matrix
-  // product; the LDS kernel for everything else
+  // the register FFT; other uniform FP32 lattices of more than 64 sites on the
+  // stacked LDS stencil (up to 64 sites the LDS kernel, which packs 2 or 4
+  // trajectories per wave, is faster); for matrix hopping the banded stencil
+  // when the coupling is banded (a dense product would mostly move zeros),
+  // else the MFMA matrix product; the LDS kernel for everything else
   if (k == TWMC_KERNEL_AUTO)
     k = stencil_ok ? TWMC_KERNEL_STENCIL64
         : fused_ok ? TWMC_KERNEL_FUSED64
-        : stack_ok ? TWMC_KERNEL_STACK
+        : (stack_ok && c->n > 64) ? TWMC_KERNEL_STACK
         : band_ok  ? TWMC_KERNEL_BAND
         : mfma_ok  ? TWMC_KERNEL_MFMA
                    : TWMC_KERNEL_GENERIC;

@@ -29,7 +29,7 @@ def spec(nx, ny, **kw):
 def test_stack_vs_fp64_oracle(nx, ny):
     sp = spec(nx, ny)
     print(f"stack {nx}x{ny}", flush=True)
-    eng = Engine(system(sp), precision=32)
+    eng = Engine(system(sp), precision=32, kernel="stack")
     res = eng.run(IDS, traj=True)
     eng.close()
     assert res.kernel == "stack"
@@ -67,7 +67,9 @@ def test_stack_resume_continues_the_stream():
 
 
 def test_auto_selection():
+    # up to 64 sites the LDS kernel (packed trajectories) is faster than the stack
     for sp, prec, want in [(spec(32, 32), 32, "stack"), (spec(64, 64, t_end=0.2), 32, "stencil64"),
+                           (spec(1, 64), 32, "generic"), (spec(8, 8), 32, "generic"), (spec(1, 66), 32, "stack"),
                            (spec(32, 32), 64, "generic"), (spec(5, 7), 32, "generic"),
                            (spec(32, 32, F_t=(np.array([0.0, 1.0]), np.ones((2, 1024), complex))), 32, "generic")]:
         eng = Engine(system(sp), precision=prec)
