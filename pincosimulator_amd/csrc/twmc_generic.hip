@@ -1,13 +1,16 @@
 // twmc_generic.hip -- LDS-resident TWMC step kernel for a lattice of any
 // size (mixed-radix FFT: radices 2, 3, 4, 5, 8 and a direct-sum pass for each
 // larger prime factor) and for matrix hopping, FP32 or FP64.  One workgroup
-// advances one trajectory at a time; a persistent grid walks the batch.  This
-// kernel is the FP64 path and the fallback for shapes the register-resident
-// kernels do not cover.
+// advances one trajectory at a time (up to four, for lattices of up to 64
+// sites); a persistent grid walks the batch.  This kernel is the FP64 path
+// and the fallback for shapes the register-resident kernels do not cover.
 //
 // Per step (src/TWMC/Solvers/TWMCBaseSolver.cpp:232-256):
 //   beta -> LDS, forward DFT (rows, then columns; Stockham, ping-pong),
 //   K product, backward DFT, /N, drift + Philox increment, update in registers.
+// With FFT hopping the steps run as nearest-neighbour launches (NN below, the
+// same linear map without the transforms) and a stored frame as a
+// capture-only launch, whose FFT gives the k-space sums.
 // Matrix hopping (src/TWMC/Solvers/TWMCLiebSolver.cpp:259-267):
 //   h_j = -i sum_i beta_i C_ij over the stored nonzeros of column j.
 #include <algorithm>
