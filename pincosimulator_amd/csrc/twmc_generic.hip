@@ -72,7 +72,8 @@ __device__ __forceinline__ double warp_sum(double v) {
 // even): split -- even sites in the first half, odd sites in the second, a
 // pair's two sites in one row, neighbour indices by a division and wrap tests.
 // 2 (ny a power of two >= 2, nx = 1 or a power of two >= 2): split, the
-// neighbour indices as masks.
+// neighbour indices as masks.  Plus 4 with a split layout when nx = 1 (no x
+// neighbours: known at compile time, so no select of the x terms).
 // BLOCK: threads per workgroup (kBlock; 64 or 128 for the step launch of a
 // lattice of up to 128 or 256 sites; 1024 for the FP64 site-uniform step
 // launch of 2049 to 4096 sites: two pairs per thread fit 128 registers, four
@@ -120,7 +121,9 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   // DB (the 1024-thread FP64 launch on a power-of-two lattice, one workgroup
   // per CU): two LDS copies of the state, each step reading one and writing
   // the new values into the other, so one barrier per step instead of two
-  constexpr bool kSplit = LAY != 0;  // split layout, pairs within rows
+  constexpr int kLay = LAY & 3;        // 0 site order, 1 / 2 split (see LAY)
+  constexpr bool kSplit = kLay != 0;  // split layout, pairs within rows
+  constexpr bool kXn = (LAY & 4) == 0;  // split layouts: the lattice has x neighbours (nx > 1)
   constexpr bool kDB = NN && kSplit && BLOCK == 1024;
   constexpr bool kBatch = kSplit && (UNIF || sizeof(T) == 4);  // neighbour reads batched
   cpx<T>* const bufD = bufA + n;
@@ -343,7 +346,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
             // site of pair rb + q + 1 (q the pair's place in its row, mod yh);
             // the x neighbours are the pairs p +- yh (mod n/2), both sites
             const int pp = p + opq, yh = g.ny >> 1;
-            if constexpr (LAY == 2) {
+            if constexpr (kLay == 2) {
               const int q = pp & (yh - 1), rb = pp - q, hm = npairs - 1;
               nbl = npairs + rb + ((q - 1) & (yh - 1));
               nbr = rb + ((q + 1) & (yh - 1));
@@ -369,7 +372,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
           if constexpr (kBatch) {
             nl = nbuf[nbl];
             nr = nbuf[nbr];
-            if (g.nx > 1) {
+            if constexpr (kXn) {
               nu[0] = nbuf[nbu];
               nd[0] = nbuf[nbd];
               nu[1] = nbuf[npairs + nbu];
@@ -392,7 +395,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
               const cpx<T> c = h == 0 ? nl : beta[2 * j];
               nsr[h] = a.re + c.re;
               nsi[h] = a.im + c.im;
-              if (g.nx > 1) {
+              if constexpr (kXn) {
                 nsr[h] += nu[h].re + nd[h].re;
                 nsi[h] += nu[h].im + nd[h].im;
               }
@@ -421,7 +424,7 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
                     const cpx<T> c = h == 0 ? nbuf[nbl] : beta[2 * j];
                     sr = a.re + c.re;
                     si = a.im + c.im;
-                    if (g.nx > 1) {
+                    if constexpr (kXn) {
                       const cpx<T> d = nbuf[h * npairs + nbu];
                       const cpx<T> e = nbuf[h * npairs + nbd];
                       sr += d.re + e.re;
@@ -746,17 +749,21 @@ const void* generic_nn_ptr_l(int ppt, int block, int tpb) {
 
 template <typename T, bool UNIF>
 const void* generic_nn_ptr_u(int ppt, int lay, int block, int tpb) {
-  return lay == 2   ? generic_nn_ptr_l<T, UNIF, 2>(ppt, block, tpb)
-         : lay == 1 ? generic_nn_ptr_l<T, UNIF, 1>(ppt, block, tpb)
-                    : generic_nn_ptr_l<T, UNIF, 0>(ppt, block, tpb);
+  switch (lay) {
+    case 6: return generic_nn_ptr_l<T, UNIF, 6>(ppt, block, tpb);
+    case 5: return generic_nn_ptr_l<T, UNIF, 5>(ppt, block, tpb);
+    case 2: return generic_nn_ptr_l<T, UNIF, 2>(ppt, block, tpb);
+    case 1: return generic_nn_ptr_l<T, UNIF, 1>(ppt, block, tpb);
+    default: return generic_nn_ptr_l<T, UNIF, 0>(ppt, block, tpb);
+  }
 }
 
 // the LDS layout of a nearest-neighbour step launch (LAY): split when a pair's
 // two sites share a row, with mask indices on power-of-two lattices
 int nn_layout(int nx, int ny) {
   auto p2 = [](int v) { return v >= 2 && (v & (v - 1)) == 0; };
-  if (p2(ny) && (nx == 1 || p2(nx))) return 2;
-  return (ny >= 2 && ny % 2 == 0) ? 1 : 0;
+  const int lay = (p2(ny) && (nx == 1 || p2(nx))) ? 2 : (ny >= 2 && ny % 2 == 0) ? 1 : 0;
+  return (lay != 0 && nx == 1) ? lay | 4 : lay;
 }
 
 const void* generic_nn_ptr(int precision, int ppt, bool unif, int lay, int block, int tpb) {
