@@ -536,6 +536,121 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
   }
 }
 
+// Capture-only launch (no steps: a stored frame of a batch whose steps run as
+// nearest-neighbour launches, or a batch's closing frame), for lattices of up
+// to 8 site pairs per thread (BLOCK threads: 1024 for lattices of 1025 to 4096
+// sites, which the step kernel's 256 threads would leave at four waves per CU,
+// else 256).  Per trajectory, as the step kernel's capture
+// does: the site means of |beta|^2 and |beta|^4, the frame dump and, with FFT
+// hopping, |DFT beta|^2.  The per-site sums over the workgroup's trajectories
+// stay in registers (each thread owns fixed sites: the pairs and the k-space
+// sites the step kernel gives it) and are added to the workgroup's partial row
+// once at the end, so every sum takes the same trajectory order as a
+// per-trajectory update of that row; the state is read and not written back.
+// LDS bytes before the state in the capture kernel: two doubles per wave
+constexpr int capture_red_bytes(int block) { return block > 256 ? block / 4 : 64; }
+
+template <typename T, int PPT, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) generic_capture_kernel(StepArgs g) {
+  static_assert(PPT <= 8 && (BLOCK == kBlock || PPT <= 2), "register sums: at most 16 (1024 threads: 4) sites per thread");
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = g.n, npairs = (n + 1) / 2, tid = threadIdx.x;
+  double* red = reinterpret_cast<double*>(smem);  // per-wave partial sums
+  cpx<T>* const bufA = reinterpret_cast<cpx<T>*>(smem + capture_red_bytes(BLOCK));
+  cpx<T>* const bufB = bufA + n;
+  cpx<T>* const twx = bufB + n;
+  cpx<T>* const twy = twx + g.nx;
+  for (int i = tid; i < g.nx; i += BLOCK) twx[i] = reinterpret_cast<const cpx<T>*>(g.twx)[i];
+  for (int i = tid; i < g.ny; i += BLOCK) twy[i] = reinterpret_cast<const cpx<T>*>(g.twy)[i];
+
+  // [site h of pair j][|z|^2, |z|^4, Re z, Im z] and the k-space sites tid + j BLOCK
+  double accr[2 * PPT][4], acck[2 * PPT];
+#pragma unroll
+  for (int j = 0; j < 2 * PPT; ++j) {
+    acck[j] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) accr[j][q] = 0.0;
+  }
+  const int wave = tid >> 6, lane = tid & 63;
+
+  for (int b = blockIdx.x; b < g.n_traj; b += gridDim.x) {
+    const cpx<T>* gbeta = reinterpret_cast<const cpx<T>*>(g.beta) + (size_t)b * n;
+    cpx<T>* out = g.dump ? reinterpret_cast<cpx<T>*>(g.traj_out) + ((size_t)b * g.n_frames + g.frame) * n : nullptr;
+    double m2 = 0.0, m4 = 0.0;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const int p = tid + BLOCK * j;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = 2 * p + h;
+        if (p < npairs && i < n) {
+          const cpx<T> z = gbeta[i];
+          bufA[i] = z;
+          const double nn = double(z.re) * double(z.re) + double(z.im) * double(z.im);
+          accr[2 * j + h][0] += nn;
+          accr[2 * j + h][1] += nn * nn;
+          accr[2 * j + h][2] += double(z.re);
+          accr[2 * j + h][3] += double(z.im);
+          m2 += nn;
+          m4 += nn * nn;
+          if (out) out[i] = z;
+        }
+      }
+    }
+    m2 = warp_sum(m2);
+    m4 = warp_sum(m4);
+    if (lane == 0) {
+      red[2 * wave] = m2;
+      red[2 * wave + 1] = m4;
+    }
+    __syncthreads();  // the staged state and the wave sums
+    if (tid == 0) {
+      double a2 = 0.0, a4 = 0.0;
+      for (int w = 0; w < BLOCK / 64; ++w) {
+        a2 += red[2 * w];
+        a4 += red[2 * w + 1];
+      }
+      g.traj_m2[(size_t)b * g.n_frames + g.frame] = a2 / double(n);
+      g.traj_m4[(size_t)b * g.n_frames + g.frame] = a4 / double(n);
+    }
+    if (g.hopping == 0) {
+      const cpx<T>* cur = fft2d<-1>(bufA, bufB, g, twx, twy);
+#pragma unroll
+      for (int j = 0; j < 2 * PPT; ++j) {
+        const int i = tid + BLOCK * j;
+        if (i < n) {
+          const cpx<T> f = cur[i];
+          acck[j] += double(f.re) * double(f.re) + double(f.im) * double(f.im);
+        }
+      }
+    }
+    __syncthreads();  // the next trajectory overwrites the LDS state and the wave sums
+  }
+
+  double* part = g.partials + (size_t)blockIdx.x * kQ * n;
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int p = tid + BLOCK * j;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = 2 * p + h;
+      if (p < npairs && i < n) {
+        part[i] += accr[2 * j + h][0];
+        part[n + i] += accr[2 * j + h][1];
+        part[3 * n + i] += accr[2 * j + h][2];
+        part[4 * n + i] += accr[2 * j + h][3];
+      }
+    }
+  }
+  if (g.hopping == 0) {
+#pragma unroll
+    for (int j = 0; j < 2 * PPT; ++j) {
+      const int i = tid + BLOCK * j;
+      if (i < n) part[2 * n + i] += acck[j];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- init
 // NoisyMatrix realisation of parameter `which` at site i for trajectory traj
 // (src/Base/NoisyMatrix.cpp:118-160), FP64.
@@ -711,9 +826,34 @@ hipError_t launch_generic_t(const StepArgs& a, int grid, int block, size_t lds, 
   return hipGetLastError();
 }
 
+// lds: the step kernel's (generic_lds_bytes), which holds the capture kernel's
+// state, twiddles and its 64 bytes of wave sums
+template <typename T, int PPT, int BLOCK>
+hipError_t launch_generic_capture_t(const StepArgs& a, int grid, size_t lds, hipStream_t s) {
+  const void* f = (const void*)generic_capture_kernel<T, PPT, BLOCK>;
+  lds += capture_red_bytes(BLOCK) - 64;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((generic_capture_kernel<T, PPT, BLOCK>), dim3(grid), dim3(BLOCK), lds, s, a);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_generic_p(const StepArgs& a, int grid, int block, size_t lds, hipStream_t s) {
   int ppt = generic_pairs_per_thread(a.n, block);
+  // a capture without steps: the capture kernel, its sums in registers
+  if (a.n_steps == 0 && a.capture && block == kBlock && ppt >= 1 && ppt <= 8) {
+    if (a.n > 1024 && a.n <= 4096)
+      return a.n > 2048 ? launch_generic_capture_t<T, 2, 1024>(a, grid, lds, s)
+                        : launch_generic_capture_t<T, 1, 1024>(a, grid, lds, s);
+    switch (ppt) {
+      case 1: return launch_generic_capture_t<T, 1, kBlock>(a, grid, lds, s);
+      case 2: return launch_generic_capture_t<T, 2, kBlock>(a, grid, lds, s);
+      default: return launch_generic_capture_t<T, 8, kBlock>(a, grid, lds, s);
+    }
+  }
   if (lds > 65536) {
     hipError_t e;
     switch (ppt) {
