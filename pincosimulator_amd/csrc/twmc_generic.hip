@@ -10,7 +10,8 @@
 //   K product, backward DFT, /N, drift + Philox increment, update in registers.
 // With FFT hopping the steps run as nearest-neighbour launches (NN below, the
 // same linear map without the transforms) and a stored frame as a
-// capture-only launch, whose FFT gives the k-space sums.
+// capture-only launch (generic_capture_kernel), whose FFT gives the k-space
+// sums.
 // Matrix hopping (src/TWMC/Solvers/TWMCLiebSolver.cpp:259-267):
 //   h_j = -i sum_i beta_i C_ij over the stored nonzeros of column j.
 #include <algorithm>
