@@ -123,6 +123,30 @@ def test_fp64_more_trajectories_than_workgroups(nx, ny, n_traj):
     assert err < TOL64
 
 
+@pytest.mark.parametrize("nx,ny,n_traj", [(64, 64, 600), (32, 48, 2000), (8, 8, 5000)])
+def test_fp64_frame_sums_over_many_trajectories_per_workgroup(nx, ny, n_traj):
+    """The capture launch with more trajectories than workgroups, each
+    workgroup summing several trajectories per site in registers (64x64 and
+    32x48: 1024 threads, two and one site pairs per thread; 8x8: 256 threads):
+    the ensemble sums and per-trajectory means equal the same estimators over
+    the stored frames, which the oracle pins trajectory by trajectory, to 1e-12
+    (FP64 sums in another order)."""
+    spec = dict(nx=nx, ny=ny, dt=0.002, gamma=1.0, J=1.0, U=0.05, omega=1.0, F=2.0, beta_init=0.0,
+                beta_sigma=0.5, frame_steps=20, seed=31, t_start=0.0, t_end=0.08)
+    ids = np.arange(n_traj, dtype=np.uint64) * 5 + 2
+    print(f"fp64 frame sums {nx}x{ny}, {n_traj} trajectories", flush=True)
+    eng = Engine(system(spec), precision=64, kernel="generic")
+    res = eng.run(ids, traj=True)
+    eng.close()
+    ref = from_trajectories(res.traj, (nx, ny))
+    errs = {k: rel_err(getattr(res, k), ref[k]) for k in ("sum_n", "sum_n2", "sum_nk", "sum_beta", "traj_m2", "traj_m4")}
+    print("  " + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
+    assert all(v < 1e-12 for v in errs.values())
+    pick = [0, n_traj // 2, n_traj - 1]
+    ora = np.array([oracle.run_trajectory(dict(spec, lattice=oracle.LAT_FFT), int(ids[j]))[0] for j in pick])
+    assert rel_err(res.traj[pick], ora) < TOL64
+
+
 def test_fp64_64x64_subset():
     spec = dict(nx=64, ny=64, dt=0.002, gamma=1.0, J=1.0, U=0.05, omega=1.0, F=2.0, beta_init=0.0,
                 beta_sigma=0.5, frame_steps=500, seed=2024, t_start=0.0, t_end=2.0)
