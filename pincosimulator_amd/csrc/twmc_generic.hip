@@ -539,21 +539,22 @@ __global__ void __launch_bounds__(BLOCK) generic_step_kernel(StepArgs g) {
 
 // Capture-only launch (no steps: a stored frame of a batch whose steps run as
 // nearest-neighbour launches, or a batch's closing frame), for lattices of up
-// to 8 site pairs per thread (BLOCK threads: 1024 for lattices of 1025 to 4096
-// sites, which the step kernel's 256 threads would leave at four waves per CU,
-// else 256).  Per trajectory, as the step kernel's capture
-// does: the site means of |beta|^2 and |beta|^4, the frame dump and, with FFT
-// hopping, |DFT beta|^2.  The per-site sums over the workgroup's trajectories
-// stay in registers (each thread owns fixed sites: the pairs and the k-space
-// sites the step kernel gives it) and are added to the workgroup's partial row
-// once at the end, so every sum takes the same trajectory order as a
-// per-trajectory update of that row; the state is read and not written back.
+// to 4096 sites.  BLOCK threads: 1024 for 1025 to 4096 sites (one or two site
+// pairs per thread, 16 waves for the FFT passes of a workgroup whose two LDS
+// state copies may be the only one on its CU), else 256 (one or two pairs).
+// Per trajectory, as the step kernel's capture does: the site means of
+// |beta|^2 and |beta|^4, the frame dump and, with FFT hopping, |DFT beta|^2.
+// The per-site sums over the workgroup's trajectories stay in registers (each
+// thread owns fixed sites: the pairs tid + BLOCK j and the k-space sites
+// tid + BLOCK j) and are added to the workgroup's partial row once at the
+// end, so every sum takes the same trajectory order as a per-trajectory update
+// of that row; the state is read and not written back.
 // LDS bytes before the state in the capture kernel: two doubles per wave
 constexpr int capture_red_bytes(int block) { return block > 256 ? block / 4 : 64; }
 
 template <typename T, int PPT, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) generic_capture_kernel(StepArgs g) {
-  static_assert(PPT <= 8 && (BLOCK == kBlock || PPT <= 2), "register sums: at most 16 (1024 threads: 4) sites per thread");
+  static_assert(PPT <= 2, "register sums: at most 4 sites per thread");
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = g.n, npairs = (n + 1) / 2, tid = threadIdx.x;
   double* red = reinterpret_cast<double*>(smem);  // per-wave partial sums
@@ -845,15 +846,11 @@ template <typename T>
 hipError_t launch_generic_p(const StepArgs& a, int grid, int block, size_t lds, hipStream_t s) {
   int ppt = generic_pairs_per_thread(a.n, block);
   // a capture without steps: the capture kernel, its sums in registers
-  if (a.n_steps == 0 && a.capture && block == kBlock && ppt >= 1 && ppt <= 8) {
-    if (a.n > 1024 && a.n <= 4096)
-      return a.n > 2048 ? launch_generic_capture_t<T, 2, 1024>(a, grid, lds, s)
-                        : launch_generic_capture_t<T, 1, 1024>(a, grid, lds, s);
-    switch (ppt) {
-      case 1: return launch_generic_capture_t<T, 1, kBlock>(a, grid, lds, s);
-      case 2: return launch_generic_capture_t<T, 2, kBlock>(a, grid, lds, s);
-      default: return launch_generic_capture_t<T, 8, kBlock>(a, grid, lds, s);
-    }
+  if (a.n_steps == 0 && a.capture && block == kBlock && a.n <= 4096) {
+    if (a.n > 2048) return launch_generic_capture_t<T, 2, 1024>(a, grid, lds, s);
+    if (a.n > 1024) return launch_generic_capture_t<T, 1, 1024>(a, grid, lds, s);
+    return ppt == 1 ? launch_generic_capture_t<T, 1, kBlock>(a, grid, lds, s)
+                    : launch_generic_capture_t<T, 2, kBlock>(a, grid, lds, s);
   }
   if (lds > 65536) {
     hipError_t e;
