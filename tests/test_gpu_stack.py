@@ -39,13 +39,19 @@ def test_stack_vs_fp64_oracle(nx, ny):
     assert err < FP32_BAR
 
 
-def test_stack_vs_lds_kernel_and_sums():
-    sp = spec(24, 24, t_end=0.4)
-    ids = np.arange(300, dtype=np.uint64) * 7 + 1
+@pytest.mark.parametrize("nx,ny,n_traj", [(24, 24, 300), (32, 48, 3000), (48, 48, 3000)])
+def test_stack_vs_lds_kernel_and_sums(nx, ny, n_traj):
+    """Frames of the stacked kernel come from the LDS kernel's capture launch:
+    256 threads at 24x24, 1024 threads with one and two site pairs per thread
+    at 32x48 and 48x48, where the batch gives each workgroup several
+    trajectories to sum in registers."""
+    sp = spec(nx, ny, t_end=0.4)
+    ids = np.arange(n_traj, dtype=np.uint64) * 7 + 1
+    print(f"stack sums {nx}x{ny}, {n_traj} trajectories", flush=True)
     a = Engine(system(sp), precision=32, kernel="stack").run(ids, traj=True)
     b = Engine(system(sp), precision=32, kernel="generic").run(ids, traj=True)
     assert rel_err(a.traj, b.traj) < 2e-5
-    ref = from_trajectories(a.traj, (24, 24))
+    ref = from_trajectories(a.traj, (nx, ny))
     for k in ("sum_n", "sum_n2", "sum_beta", "traj_m2", "traj_m4"):
         assert rel_err(getattr(a, k), ref[k]) < 1e-9, k
     assert rel_err(a.sum_nk, ref["sum_nk"]) < 1e-5
